@@ -1,0 +1,144 @@
+/*
+ * mm_hermite.h -- C ABI of the MI355X Bargmann->Fock library (libmmhip.so).
+ *
+ * Plain pointers and sizes only.  Complex numbers are interleaved
+ * (re, im) float64 pairs, i.e. complex128.  Every array pointer is a DEVICE
+ * pointer unless the comment says "host".  `stream` is a hipStream_t passed
+ * as void* (NULL = the default stream).  All entry points are stream-ordered
+ * and never synchronise the host, except the mm_plan_* set-up calls.
+ *
+ * Return value: MM_OK (0) or a negative status; mm_last_error() then gives a
+ * thread-local message.  No entry point frees caller memory.  Workspace
+ * buffers are caller-provided: query the size first, pass at least that many
+ * bytes (16-byte aligned).
+ */
+#ifndef MM_HERMITE_H
+#define MM_HERMITE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_OK 0
+#define MM_ERR_INVALID (-1)     /* bad shape, D, sizes, null pointer            */
+#define MM_ERR_HIP (-2)         /* a HIP runtime call failed                     */
+#define MM_ERR_WORKSPACE (-3)   /* workspace missing or smaller than required    */
+#define MM_ERR_UNSUPPORTED (-4) /* valid input outside what the kernels support  */
+
+#define MM_MAX_DIMS 8        /* D <= 8 indices per triple                       */
+#define MM_MAX_CUTOFF 100000 /* each shape entry < 100000: the reference's SQRT
+                                table length, core.py:22                        */
+
+/* Message for the last failing call on this thread (never NULL). */
+const char *mm_last_error(void);
+
+/* Library version string. */
+const char *mm_version(void);
+
+/* Upload the per-device constant tables (sqrt(n) and 1/sqrt(n), n < 100000)
+ * for `device`.  Called implicitly by every entry point; idempotent. */
+int mm_init(int device);
+
+/* ------------------------------------------------------------------ forward */
+
+/* Workspace bytes needed by mm_hermite_fwd for this shape (0 for most). */
+size_t mm_hermite_fwd_workspace(const int64_t *shape /* host, D entries */, int D, int stable);
+
+/* G = renormalized multidimensional Hermite tensor of (A, b, c) on `shape`,
+ * C order (last index fastest).  stable = 0: first-pivot recurrence;
+ * stable = 1: average over all pivots.
+ * Replaces vanilla_numba / stable_numba
+ *   (mrmustard/math/lattice/strategies/vanilla/core.py:25-124, :127-213)
+ * as bound by BackendNumpy.hermite_renormalized (math/backend_numpy.py:381-392).
+ * A: D*D complex (row-major), b: D complex, c: 1 complex, G: prod(shape) complex. */
+int mm_hermite_fwd(const double *A, const double *b, const double *c, const int64_t *shape, int D,
+                   int stable, double *G, void *workspace, size_t ws_bytes, void *stream);
+
+/* Uniform batch: B triples sharing one shape.  A: B*D*D, b: B*D, c: B,
+ * G: B*prod(shape) complex.  Replaces vanilla_batch_numba
+ *   (vanilla/batch.py:27-61) as bound by BackendNumpy.hermite_renormalized_batched
+ *   (math/backend_numpy.py:394-403). */
+size_t mm_hermite_batch_fwd_workspace(int64_t B, const int64_t *shape, int D, int stable);
+int mm_hermite_batch_fwd(const double *A, const double *b, const double *c, int64_t B,
+                         const int64_t *shape, int D, int stable, double *G, void *workspace,
+                         size_t ws_bytes, void *stream);
+
+/* ---------------------------------------------------------------------- VJP */
+
+/* Holomorphic vector-Jacobian product of the vanilla forward (no conjugation
+ * of dLdG): dA (D*D, symmetrised (M + M^T)/2), db (D), dc (1).
+ * Replaces vanilla_vjp_numba (vanilla/gradients.py:25-82), the backward of
+ * hermite_renormalized_jax (math/jax_vjps/hermite.py:86-102). */
+size_t mm_hermite_vjp_workspace(const int64_t *shape, int D);
+int mm_hermite_vjp(const double *G, const double *c, const double *dLdG, const int64_t *shape, int D,
+                   double *dA, double *db, double *dc, void *workspace, size_t ws_bytes, void *stream);
+
+/* Uniform-batch VJP.  Replaces vanilla_batch_vjp_numba (vanilla/gradients.py:85-116). */
+size_t mm_hermite_batch_vjp_workspace(int64_t B, const int64_t *shape, int D);
+int mm_hermite_batch_vjp(const double *G, const double *c, const double *dLdG, int64_t B,
+                         const int64_t *shape, int D, double *dA, double *db, double *dc,
+                         void *workspace, size_t ws_bytes, void *stream);
+
+/* Fused forward + VJP in one sweep when dL/dG is known up front: the
+ * composition vanilla_numba -> vanilla_vjp_numba (core.py:25, gradients.py:25). */
+size_t mm_hermite_fused_workspace(const int64_t *shape, int D);
+int mm_hermite_fused(const double *A, const double *b, const double *c, const double *dLdG,
+                     const int64_t *shape, int D, double *G, double *dA, double *db, double *dc,
+                     void *workspace, size_t ws_bytes, void *stream);
+
+/* Uniform-batch fused forward + VJP: the composition vanilla_batch_numba ->
+ * vanilla_batch_vjp_numba (batch.py:27, gradients.py:85) in one sweep per triple. */
+size_t mm_hermite_batch_fused_workspace(int64_t B, const int64_t *shape, int D);
+int mm_hermite_batch_fused(const double *A, const double *b, const double *c, const double *dLdG,
+                           int64_t B, const int64_t *shape, int D, double *G, double *dA,
+                           double *db, double *dc, void *workspace, size_t ws_bytes, void *stream);
+
+/* ------------------------------------------------- work-queue batch (plans)
+ *
+ * A plan describes n triples of uneven mode count D_k <= 8 and shape_k.
+ * Packed layouts (complex elements, triples in the order given):
+ *   A, dA: concatenation of D_k*D_k blocks;  b, db: of D_k blocks;
+ *   c, dc: n entries;                         G, dLdG: of prod(shape_k) blocks.
+ * mm_plan_create sorts the triples largest-first into a device work queue
+ * consumed by one persistent kernel (blocks take large triples, single waves
+ * take small ones, single lanes take one-index chains).  It allocates the
+ * plan's own device tables and synchronises the device once.
+ * The reference has no uneven batch: its counterpart is a loop of
+ * vanilla_numba calls, or vanilla_batch_numba (batch.py:27) when all shapes
+ * agree.  mm_plan_run launches on `stream`; it may be captured in a graph.
+ */
+typedef struct mm_plan mm_plan;
+
+typedef struct {
+    int block_threads;   /* threads per workgroup: 256, 512 or 1024 (0 = auto)  */
+    int blocks_per_cu;   /* persistent workgroups per compute unit (0 = auto)    */
+    int wave_max_slab;   /* largest axis-0 slab (prod(shape[1:])) given to a single
+                            wave instead of a workgroup (0 = auto, -1 = never)    */
+    int reserved;
+} mm_tune;
+
+int mm_plan_create(int64_t n, const int32_t *dims /* host, n */, const int64_t *shapes /* host, n*8 */,
+                   int fused, const mm_tune *tune /* may be NULL */, mm_plan **plan);
+int mm_plan_destroy(mm_plan *plan);
+/* Totals of the plan (host): number of amplitudes and of complex elements in A and b. */
+int mm_plan_info(const mm_plan *plan, int64_t *n_amplitudes, int64_t *a_elems, int64_t *b_elems);
+/* How the plan splits its triples (host, 8 entries): workgroup triples, wave
+ * triples, lane triples, triples run outside the persistent kernel, grid,
+ * threads per workgroup, dynamic LDS bytes per workgroup, LDS bytes per wave. */
+int mm_plan_describe(const mm_plan *plan, int64_t *out);
+/* dLdG, dA, db, dc must be NULL for a plan created with fused = 0. */
+int mm_plan_run(const mm_plan *plan, const double *A, const double *b, const double *c,
+                const double *dLdG, double *G, double *dA, double *db, double *dc, void *stream);
+/* As mm_plan_run, but through a HIP graph that the plan captures on first use
+ * (and re-captures when the buffer pointers change). */
+int mm_plan_run_graph(mm_plan *plan, const double *A, const double *b, const double *c,
+                      const double *dLdG, double *G, double *dA, double *db, double *dc, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MM_HERMITE_H */

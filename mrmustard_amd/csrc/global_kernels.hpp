@@ -1,0 +1,287 @@
+// Kernels that work on G in global memory: the per-slab forward for lattices
+// whose slabs do not fit in LDS, the closed-form VJP reduction, and the
+// all-pivot ("stable") recurrence swept by index-sum wavefronts.
+#pragma once
+
+#include "common.hpp"
+#include "jobs.hpp"
+
+namespace mm {
+
+struct Geom {
+    int32_t D;
+    int32_t shape[8];
+    int64_t stride[8];
+    int64_t N;
+};
+
+// Mixed-radix digits of x over axes [lo, D).
+template <int D>
+__device__ __forceinline__ void digits_of(int64_t x, const Geom &g, int lo, int (&k)[D]) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        if (j < lo) {
+            k[j] = 0;
+            continue;
+        }
+        int64_t dj = x / g.stride[j];
+        x -= dj * g.stride[j];
+        k[j] = int(dj);
+    }
+}
+
+// k += inc (both as digits over axes [lo, D)); returns false on overflow past axis lo.
+template <int D>
+__device__ __forceinline__ bool digits_add(int (&k)[D], const int (&inc)[D], const Geom &g, int lo) {
+    int carry = 0;
+#pragma unroll
+    for (int j = D - 1; j >= 0; --j) {
+        if (j < lo) break;
+        int v = k[j] + inc[j] + carry;
+        carry = v >= g.shape[j];
+        k[j] = carry ? v - g.shape[j] : v;
+    }
+    return carry == 0;
+}
+
+// ------------------------------------------------------------ per-slab forward
+// Slab k_d = m of the sub-lattice {k_0..k_{d-1} = 0}: every point has pivot
+// axis d (core.py:88-104 with i = d) and reads only slabs m-1 and m-2.
+struct SlabArgs {
+    Geom g;
+    const double2 *A, *b;
+    double2 *G;
+    const double *sq, *rsq;
+    int d, m;
+};
+
+template <int D>
+__global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
+    const int d = a.d, m = a.m;
+    const int64_t Sd = a.g.stride[d];
+    const int64_t q0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t inc = int64_t(gridDim.x) * blockDim.x;
+    if (q0 >= Sd) return;
+    int k[D], ki[D];
+    digits_of<D>(q0, a.g, d + 1, k);
+    digits_of<D>(inc < Sd ? inc : 0, a.g, d + 1, ki);
+    const double2 bd = a.b[d];
+    const double2 cdd = cscale_promoted(a.A[d * D + d], a.sq[m > 0 ? m - 1 : 0]);
+    const double sm = a.sq[m], rm = a.rsq[m];
+    double2 Adj[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) Adj[j] = (j > d) ? a.A[d * D + j] : make_double2(0.0, 0.0);
+    double2 *G = a.G;
+    const int64_t base = int64_t(m) * Sd;
+    for (int64_t q = q0; q < Sd; q += inc) {
+        const int64_t f = base + q;
+        double2 v = cmul(bd, G[f - Sd]);
+        if (m >= 2) v = cadd(v, cmul(cdd, G[f - 2 * Sd]));
+#pragma unroll
+        for (int j = 1; j < D; ++j) {
+            if (j <= d) continue;
+            if (k[j] > 0) v = cadd(v, cmul(cscale_promoted(Adj[j], a.sq[k[j]]), G[f - Sd - a.g.stride[j]]));
+        }
+        G[f] = cdiv_by(v, sm, rm);
+        if (q + inc < Sd) digits_add<D>(k, ki, a.g, d + 1);
+    }
+}
+
+// ------------------------------------------------------------------- the VJP
+// gradients.py:64-80 as a parallel reduction: each thread accumulates the
+// terms of its points (fused multiply-adds), a block reduces through shuffles
+// and LDS, and writes one partial per block; vjp_finalize sums the partials
+// of a triple in block order (deterministic) and symmetrises.
+struct VjpArgs {
+    Geom g;
+    const double2 *G, *gg;
+    double2 *partial;
+    const double *sq;
+    int nchunks;
+};
+
+template <int D>
+__global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
+    constexpr int NA = D * (D + 1) / 2 + D + 1;
+    constexpr int NT = D * (D + 1) / 2;
+    const int64_t t = blockIdx.y;
+    const double2 *G = a.G + t * a.g.N;
+    const double2 *gg = a.gg + t * a.g.N;
+    double2 acc[NA];
+#pragma unroll
+    for (int e = 0; e < NA; ++e) acc[e] = make_double2(0.0, 0.0);
+    const int64_t f0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t inc = int64_t(gridDim.x) * blockDim.x;
+    if (f0 < a.g.N) {
+        int k[D], ki[D];
+        digits_of<D>(f0, a.g, 0, k);
+        digits_of<D>(inc < a.g.N ? inc : 0, a.g, 0, ki);
+        for (int64_t f = f0; f < a.g.N; f += inc) {
+            const double2 gk = gg[f];
+            acc[NA - 1] = cfma(G[f], gk, acc[NA - 1]);
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                const int kk = k[i];
+                if (kk == 0) continue;
+                const double si = a.sq[kk];
+                const int64_t p = f - a.g.stride[i];
+                acc[NT + i] = caxpy(si, cmul(G[p], gk), acc[NT + i]);
+                const int tii = i * D - i * (i - 1) / 2;
+                if (kk > 1) acc[tii] = caxpy(0.5 * si * a.sq[kk - 1], cmul(G[p - a.g.stride[i]], gk), acc[tii]);
+#pragma unroll
+                for (int j = i + 1; j < D; ++j) {
+                    if (k[j] > 0) acc[tii + (j - i)] = caxpy(si * a.sq[k[j]], cmul(G[p - a.g.stride[j]], gk), acc[tii + (j - i)]);
+                }
+            }
+            if (f + inc < a.g.N) digits_add<D>(k, ki, a.g, 0);
+        }
+    }
+    __shared__ double2 red[(256 / MM_WAVE) * NA];
+#pragma unroll
+    for (int e = 0; e < NA; ++e) acc[e] = wave_sum(acc[e]);
+    const int lane = threadIdx.x & (MM_WAVE - 1), w = threadIdx.x / MM_WAVE;
+    if (lane == 0) {
+#pragma unroll
+        for (int e = 0; e < NA; ++e) red[w * NA + e] = acc[e];
+    }
+    __syncthreads();
+    if (threadIdx.x < NA) {
+        double2 s = red[threadIdx.x];
+        for (int ww = 1; ww < int(blockDim.x / MM_WAVE); ++ww) s = cadd(s, red[ww * NA + threadIdx.x]);
+        a.partial[(t * a.nchunks + blockIdx.x) * NA + threadIdx.x] = s;
+    }
+}
+
+// One block per triple.  npart partial rows of NA sums each (the VJP kernel's
+// chunks, optionally followed by extra rows from other kernels).
+template <int D>
+__global__ __launch_bounds__(64) void vjp_finalize(const double2 *partial, int npart, const double2 *c, double2 *dA, double2 *db,
+                                                   double2 *dc) {
+    constexpr int NA = D * (D + 1) / 2 + D + 1;
+    constexpr int NT = D * (D + 1) / 2;
+    __shared__ double2 s[NA];
+    const int64_t t = blockIdx.x;
+    const double2 *p = partial + t * npart * NA;
+    if (threadIdx.x < NA) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int r = 0; r < npart; ++r) acc = cadd(acc, p[r * NA + threadIdx.x]);
+        s[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x < D * D) {
+        const int i = threadIdx.x / D, j = threadIdx.x % D;
+        const int lo = i < j ? i : j, hi = i < j ? j : i;
+        const double2 m = s[lo * D - lo * (lo - 1) / 2 + (hi - lo)];
+        dA[t * D * D + threadIdx.x] = (i == j) ? m : make_double2(m.x * 0.5, m.y * 0.5);
+    }
+    if (threadIdx.x < D) db[t * D + threadIdx.x] = s[NT + threadIdx.x];
+    if (threadIdx.x == 0) dc[t] = cdiv_py(s[NA - 1], c[t]);
+}
+
+// ------------------------------------------------------------- stable sweep
+// core.py:183-211.  All pivots of a point lie on the previous index-sum
+// wavefront, so the lattice is swept by |k| = n.  A "row" fixes k_0..k_{D-2};
+// on wavefront n row r holds exactly one point, k_{D-1} = n - |row r|, when
+// that is inside the box.  The row table stores each row's digits (uint16).
+struct StableArgs {
+    Geom g;
+    const double2 *A, *b, *c;
+    double2 *G;
+    const uint16_t *rows;  // nrows x 8 digits
+    int64_t nrows;
+    const double *sq, *rsq;
+    int n;                 // wavefront (per-wavefront launch)
+    int64_t tstride_a, tstride_b, tstride_g;  // batch strides (complex elements)
+};
+
+__global__ void row_table_kernel(Geom g, uint16_t *rows, int64_t nrows) {
+    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int D = g.D;
+    const int64_t slast = g.shape[D - 1];
+    int64_t x = r * slast;
+    uint16_t *out = rows + r * 8;
+    for (int j = 0; j < 8; ++j) {
+        if (j < D - 1) {
+            int64_t dj = x / g.stride[j];
+            x -= dj * g.stride[j];
+            out[j] = uint16_t(dj);
+        } else {
+            out[j] = 0;
+        }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void stable_point(const StableArgs &a, const double2 *A, const double2 *b, double2 *G, const int (&k)[D],
+                                             int64_t f) {
+    int num = 0;
+    double2 vals = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        if (k[i] == 0) continue;
+        ++num;
+        const int64_t pivot = f - a.g.stride[i];
+        double2 val = cmul(b[i], G[pivot]);
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            if (j >= i) break;
+            if (k[j] > 0) val = cadd(val, cmul(cscale_promoted(A[i * D + j], a.sq[k[j]]), G[pivot - a.g.stride[j]]));
+        }
+        if (k[i] >= 2) val = cadd(val, cmul(cscale_promoted(A[i * D + i], a.sq[k[i] - 1]), G[pivot - a.g.stride[i]]));
+#pragma unroll
+        for (int j = i + 1; j < D; ++j) {
+            if (k[j] > 0) val = cadd(val, cmul(cscale_promoted(A[i * D + j], a.sq[k[j]]), G[pivot - a.g.stride[j]]));
+        }
+        vals = cadd(vals, cdiv_by(val, a.sq[k[i]], a.rsq[k[i]]));
+    }
+    G[f] = cdiv_py(vals, make_double2(double(num), 0.0));
+}
+
+template <int D>
+__device__ __forceinline__ void stable_row(const StableArgs &a, const double2 *A, const double2 *b, double2 *G, int64_t r, int n) {
+    const uint16_t *rd = a.rows + r * 8;
+    int k[D];
+    int p = 0;
+#pragma unroll
+    for (int j = 0; j < D - 1; ++j) {
+        k[j] = rd[j];
+        p += k[j];
+    }
+    const int kl = n - p;
+    if (kl < 0 || kl >= a.g.shape[D - 1]) return;
+    k[D - 1] = kl;
+    stable_point<D>(a, A, b, G, k, r * a.g.shape[D - 1] + kl);
+}
+
+// One wavefront of one triple per launch (large lattices).
+template <int D>
+__global__ __launch_bounds__(256) void stable_wavefront_kernel(StableArgs a) {
+    const int64_t r0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (int64_t r = r0; r < a.nrows; r += int64_t(gridDim.x) * blockDim.x) stable_row<D>(a, a.A, a.b, a.G, r, a.n);
+}
+
+// Whole sweep of one triple per workgroup (small lattices, batches).
+template <int D>
+__global__ __launch_bounds__(1024) void stable_block_kernel(StableArgs a) {
+    const int64_t t = blockIdx.x;
+    const double2 *A = a.A + t * a.tstride_a;
+    const double2 *b = a.b + t * a.tstride_b;
+    double2 *G = a.G + t * a.tstride_g;
+    if (threadIdx.x == 0) G[0] = a.c[t];
+    __syncthreads();
+    int nmax = 0;
+    for (int j = 0; j < D; ++j) nmax += a.g.shape[j] - 1;
+    for (int n = 1; n <= nmax; ++n) {
+        for (int64_t r = threadIdx.x; r < a.nrows; r += blockDim.x) stable_row<D>(a, A, b, G, r, n);
+        __syncthreads();
+    }
+}
+
+__global__ void set_first_kernel(double2 *G, const double2 *c) { G[0] = c[0]; }
+
+__global__ void zero_ints_kernel(int *p, int n) {
+    if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
+}
+
+}  // namespace mm

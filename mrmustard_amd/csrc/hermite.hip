@@ -1,0 +1,835 @@
+// libmmhip: C ABI over the MI355X Bargmann->Fock kernels.
+//
+// Host-side planning: validation (the ValueError conditions of
+// backend_manager.py:674-689 plus the shape/D checks the reference leaves to
+// chance), choice of execution path per triple, kernel launches.  Device
+// state is one sqrt / 1/sqrt table per device, created on first use.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mm_hermite.h"
+#include "common.hpp"
+#include "global_kernels.hpp"
+#include "jobs.hpp"
+#include "lds_kernel.hpp"
+
+using namespace mm;
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return fail(MM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK(what)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = hipGetLastError();                                                      \
+        if (e_ != hipSuccess) return fail(MM_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevState {
+    double *sq = nullptr, *rsq = nullptr;
+    int max_lds = 65536;
+    int num_cu = 1;
+};
+
+std::mutex g_mu;
+std::map<int, DevState> g_dev;
+
+int device_state(DevState **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_dev.find(dev);
+    if (it != g_dev.end()) {
+        *out = &it->second;
+        return MM_OK;
+    }
+    DevState st;
+    std::vector<double> sq(MM_MAX_CUTOFF), rsq(MM_MAX_CUTOFF);
+    for (int n = 0; n < MM_MAX_CUTOFF; ++n) {
+        sq[n] = std::sqrt(double(n));  // correctly rounded, as np.sqrt (core.py:22)
+        rsq[n] = n ? 1.0 / sq[n] : 0.0;
+    }
+    HIP_TRY(hipMalloc(&st.sq, sizeof(double) * MM_MAX_CUTOFF));
+    HIP_TRY(hipMalloc(&st.rsq, sizeof(double) * MM_MAX_CUTOFF));
+    HIP_TRY(hipMemcpy(st.sq, sq.data(), sizeof(double) * MM_MAX_CUTOFF, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(st.rsq, rsq.data(), sizeof(double) * MM_MAX_CUTOFF, hipMemcpyHostToDevice));
+    HIP_TRY(hipDeviceGetAttribute(&st.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    HIP_TRY(hipDeviceGetAttribute(&st.num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    st.max_lds -= 256;  // headroom for the kernels' static LDS
+    auto ins = g_dev.emplace(dev, st);
+    *out = &ins.first->second;
+    return MM_OK;
+}
+
+// ------------------------------------------------------------------ geometry
+
+struct Shape {
+    int D = 0;
+    int32_t s[8] = {0};
+    int64_t st[8] = {0};
+    int64_t N = 0;
+};
+
+int make_shape(const int64_t *shape, int D, Shape *out) {
+    if (!shape) return fail(MM_ERR_INVALID, "shape pointer is NULL");
+    if (D < 1 || D > MM_MAX_DIMS) return fail(MM_ERR_INVALID, "D=%d outside [1, %d]", D, MM_MAX_DIMS);
+    Shape S;
+    S.D = D;
+    int64_t n = 1;
+    for (int i = 0; i < D; ++i) {
+        if (shape[i] < 1 || shape[i] >= MM_MAX_CUTOFF)
+            return fail(MM_ERR_INVALID, "shape[%d]=%lld outside [1, %d)", i, (long long)shape[i], MM_MAX_CUTOFF);
+        S.s[i] = int32_t(shape[i]);
+        if (n > (int64_t(1) << 40) / shape[i]) return fail(MM_ERR_INVALID, "lattice too large");
+        n *= shape[i];
+    }
+    S.st[D - 1] = 1;
+    for (int i = D - 1; i > 0; --i) S.st[i - 1] = S.st[i] * S.s[i];
+    S.N = n;
+    *out = S;
+    return MM_OK;
+}
+
+Geom geom_of(const Shape &S) {
+    Geom g;
+    memset(&g, 0, sizeof(g));
+    g.D = S.D;
+    for (int i = 0; i < 8; ++i) {
+        g.shape[i] = i < S.D ? S.s[i] : 1;
+        g.stride[i] = i < S.D ? S.st[i] : 1;
+    }
+    g.N = S.N;
+    return g;
+}
+
+// Job template for the sub-lattice swept in LDS from level `top` down.
+Job make_job(const Shape &S, int top, int kind) {
+    Job J;
+    memset(&J, 0, sizeof(J));
+    J.D = S.D;
+    J.kind = kind;
+    J.top = top;
+    J.S = int32_t(S.st[top]);
+    J.nacc = nacc_of(S.D);
+    int smax = 1;
+    for (int j = top; j < S.D; ++j) smax = std::max(smax, int(S.s[j]));
+    J.smax = smax;
+    for (int i = 0; i < 8; ++i) {
+        J.shape[i] = i < S.D ? S.s[i] : 1;
+        J.stride[i] = i < S.D ? S.st[i] : 1;
+    }
+    int bit = 0;
+    for (int j = S.D - 1; j > top; --j) {
+        int w = 0;
+        while ((1 << w) < S.s[j]) ++w;
+        J.kbit[j] = bit;
+        J.kmask[j] = (1 << w) - 1;
+        bit += w;
+    }
+    return J;
+}
+
+int64_t lds_bytes(const Shape &S, int top, bool fused, int nwaves) {
+    int smax = 1;
+    for (int j = top; j < S.D; ++j) smax = std::max(smax, int(S.s[j]));
+    return lds_layout(S.D, top, int(S.st[top]), smax, fused ? 1 : 0, nwaves).total;
+}
+
+bool packed_bits_ok(const Shape &S, int top) {
+    int bits = 0;
+    for (int j = S.D - 1; j > top; --j) {
+        int w = 0;
+        while ((1 << w) < S.s[j]) ++w;
+        bits += w;
+    }
+    return bits <= 32;
+}
+
+int team_threads(int64_t slab, int maxt) {
+    int64_t t = ((slab + 63) / 64) * 64;
+    t = std::max<int64_t>(64, std::min<int64_t>(t, maxt));
+    return int(t);
+}
+
+bool lds_kernel_dmax8(int D) { return D > 3; }
+int lds_maxt(int D, bool fused) { return fused ? (lds_kernel_dmax8(D) ? 256 : 512) : 1024; }
+
+// Smallest top level whose sweep fits in LDS with `threads` threads (D >= 2).
+int choose_top(const Shape &S, bool fused, int max_lds, int maxt, int *threads) {
+    for (int top = 0; top < S.D; ++top) {
+        int T = team_threads(S.st[top], maxt);
+        if (S.st[top] > int64_t(8) * 1024 * 1024) continue;
+        if (!packed_bits_ok(S, top)) continue;
+        if (lds_bytes(S, top, fused, T / MM_WAVE) <= max_lds) {
+            *threads = T;
+            return top;
+        }
+    }
+    *threads = 64;
+    return S.D - 1;
+}
+
+template <typename K>
+void raise_lds_limit(K kernel, int bytes) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+// Launch the LDS kernel instantiation that covers D / fused.
+int launch_lds(const LdsArgs &a, int D, bool fused, int grid, int threads, int64_t shmem, hipStream_t s) {
+    if (lds_kernel_dmax8(D)) {
+        if (fused) {
+            raise_lds_limit(lds_kernel<8, true, 256>, int(shmem));
+            hipLaunchKernelGGL((lds_kernel<8, true, 256>), dim3(grid), dim3(threads), size_t(shmem), s, a);
+        } else {
+            raise_lds_limit(lds_kernel<8, false, 1024>, int(shmem));
+            hipLaunchKernelGGL((lds_kernel<8, false, 1024>), dim3(grid), dim3(threads), size_t(shmem), s, a);
+        }
+    } else {
+        if (fused) {
+            raise_lds_limit(lds_kernel<3, true, 512>, int(shmem));
+            hipLaunchKernelGGL((lds_kernel<3, true, 512>), dim3(grid), dim3(threads), size_t(shmem), s, a);
+        } else {
+            raise_lds_limit(lds_kernel<3, false, 1024>, int(shmem));
+            hipLaunchKernelGGL((lds_kernel<3, false, 1024>), dim3(grid), dim3(threads), size_t(shmem), s, a);
+        }
+    }
+    LAUNCH_CHECK("lds_kernel");
+    return MM_OK;
+}
+
+Bases bases(const double *A, const double *b, const double *c, const double *g, double *G, double *dA, double *db, double *dc,
+            double *partial) {
+    Bases P;
+    P.A = reinterpret_cast<const double2 *>(A);
+    P.b = reinterpret_cast<const double2 *>(b);
+    P.c = reinterpret_cast<const double2 *>(c);
+    P.g = reinterpret_cast<const double2 *>(g);
+    P.G = reinterpret_cast<double2 *>(G);
+    P.dA = reinterpret_cast<double2 *>(dA);
+    P.db = reinterpret_cast<double2 *>(db);
+    P.dc = reinterpret_cast<double2 *>(dc);
+    P.partial = reinterpret_cast<double2 *>(partial);
+    return P;
+}
+
+// --------------------------------------------------------------- slab path
+
+int launch_slab(const SlabArgs &a, int D, int64_t Sd, int num_cu, hipStream_t s) {
+    int64_t blocks = (Sd + 255) / 256;
+    int64_t cap = int64_t(num_cu) * 16;
+    int grid = int(std::max<int64_t>(1, std::min(blocks, cap)));
+    switch (D) {
+#define CASE(DD) \
+    case DD: hipLaunchKernelGGL((fwd_slab_kernel<DD>), dim3(grid), dim3(256), 0, s, a); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad D");
+    }
+    LAUNCH_CHECK("fwd_slab_kernel");
+    return MM_OK;
+}
+
+// ------------------------------------------------------------------ VJP path
+
+int vjp_chunks(int64_t N, int64_t B) {
+    int64_t per = 256 * 8;
+    int64_t want = (N + per - 1) / per;
+    int64_t cap = std::max<int64_t>(1, 8192 / std::max<int64_t>(B, 1));
+    return int(std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(cap, 4096))));
+}
+
+size_t vjp_ws_bytes(const Shape &S, int64_t B) {
+    return size_t(B) * vjp_chunks(S.N, B) * nacc_of(S.D) * 16 + 256;
+}
+
+int launch_vjp(const Shape &S, int64_t B, const double *G, const double *g, const double *c, double *dA, double *db, double *dc,
+               void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s) {
+    if (!ws || ws_bytes < vjp_ws_bytes(S, B))
+        return fail(MM_ERR_WORKSPACE, "VJP workspace of %zu bytes needed, %zu given", vjp_ws_bytes(S, B), ws_bytes);
+    VjpArgs a;
+    a.g = geom_of(S);
+    a.G = reinterpret_cast<const double2 *>(G);
+    a.gg = reinterpret_cast<const double2 *>(g);
+    a.partial = reinterpret_cast<double2 *>(ws);
+    a.sq = ds->sq;
+    a.nchunks = vjp_chunks(S.N, B);
+    dim3 grid(a.nchunks, unsigned(B));
+    const double2 *c2 = reinterpret_cast<const double2 *>(c);
+    double2 *dA2 = reinterpret_cast<double2 *>(dA), *db2 = reinterpret_cast<double2 *>(db), *dc2 = reinterpret_cast<double2 *>(dc);
+    switch (S.D) {
+#define CASE(DD)                                                                                               \
+    case DD:                                                                                                   \
+        hipLaunchKernelGGL((vjp_kernel<DD>), grid, dim3(256), 0, s, a);                                       \
+        LAUNCH_CHECK("vjp_kernel");                                                                            \
+        hipLaunchKernelGGL((vjp_finalize<DD>), dim3(unsigned(B)), dim3(64), 0, s, a.partial, a.nchunks, c2, dA2, db2, dc2); \
+        LAUNCH_CHECK("vjp_finalize");                                                                          \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad D");
+    }
+    return MM_OK;
+}
+
+// -------------------------------------------------------------- stable path
+
+size_t stable_ws_bytes(const Shape &S) { return size_t(S.N / S.s[S.D - 1]) * 16 + 256; }
+
+int stable_check(const Shape &S) {
+    for (int j = 0; j + 1 < S.D; ++j)
+        if (S.s[j] > 65535) return fail(MM_ERR_UNSUPPORTED, "stable: shape[%d]=%d > 65535", j, S.s[j]);
+    return MM_OK;
+}
+
+int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, const double *c, double *G, void *ws, size_t ws_bytes,
+                  const DevState *ds, hipStream_t s) {
+    int rc = stable_check(S);
+    if (rc) return rc;
+    if (!ws || ws_bytes < stable_ws_bytes(S))
+        return fail(MM_ERR_WORKSPACE, "stable workspace of %zu bytes needed, %zu given", stable_ws_bytes(S), ws_bytes);
+    StableArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = geom_of(S);
+    a.nrows = S.N / S.s[S.D - 1];
+    a.rows = reinterpret_cast<uint16_t *>(ws);
+    a.sq = ds->sq;
+    a.rsq = ds->rsq;
+    a.tstride_a = int64_t(S.D) * S.D;
+    a.tstride_b = S.D;
+    a.tstride_g = S.N;
+    hipLaunchKernelGGL(row_table_kernel, dim3(unsigned((a.nrows + 255) / 256)), dim3(256), 0, s, a.g,
+                       reinterpret_cast<uint16_t *>(ws), a.nrows);
+    LAUNCH_CHECK("row_table_kernel");
+    int nmax = 0;
+    for (int j = 0; j < S.D; ++j) nmax += S.s[j] - 1;
+    const bool small = S.N <= (int64_t(1) << 18);
+    for (int64_t t = 0; t < B; ++t) {
+        a.A = reinterpret_cast<const double2 *>(A) + (small ? 0 : t * a.tstride_a);
+        a.b = reinterpret_cast<const double2 *>(b) + (small ? 0 : t * a.tstride_b);
+        a.c = reinterpret_cast<const double2 *>(c) + (small ? 0 : t);
+        a.G = reinterpret_cast<double2 *>(G) + (small ? 0 : t * a.tstride_g);
+        if (small) {
+            switch (S.D) {
+#define CASE(DD) \
+    case DD: hipLaunchKernelGGL((stable_block_kernel<DD>), dim3(unsigned(B)), dim3(1024), 0, s, a); break;
+                CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+            }
+            LAUNCH_CHECK("stable_block_kernel");
+            break;  // one launch covers the whole batch
+        }
+        hipLaunchKernelGGL(set_first_kernel, dim3(1), dim3(1), 0, s, a.G, a.c);
+        LAUNCH_CHECK("set_first_kernel");
+        int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((a.nrows + 255) / 256, int64_t(ds->num_cu) * 16));
+        for (int n = 1; n <= nmax; ++n) {
+            a.n = n;
+            switch (S.D) {
+#define CASE(DD) \
+    case DD: hipLaunchKernelGGL((stable_wavefront_kernel<DD>), dim3(unsigned(blocks)), dim3(256), 0, s, a); break;
+                CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+            }
+            LAUNCH_CHECK("stable_wavefront_kernel");
+        }
+    }
+    return MM_OK;
+}
+
+// ------------------------------------------------------------ vanilla paths
+
+// Workspace of one vanilla forward (+ fused VJP when `fused`).
+size_t fwd_ws_bytes(const Shape &S, bool fused, const DevState *ds) {
+    if (!fused) return 0;
+    int T = 64;
+    int top = (S.D == 1) ? 0 : choose_top(S, true, ds->max_lds, lds_maxt(S.D, true), &T);
+    if (S.D == 1 || top == 0) return 0;
+    return vjp_ws_bytes(S, 1);
+}
+
+// Forward of B triples sharing shape S (uniform strides); fused adds the VJP.
+int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b, const double *c, const double *g, double *G,
+                    double *dA, double *db, double *dc, void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s) {
+    const bool fused = g != nullptr;
+    LdsArgs a;
+    memset(&a, 0, sizeof(a));
+    a.P = bases(A, b, c, g, G, dA, db, dc, nullptr);
+    a.T.sq = ds->sq;
+    a.T.rsq = ds->rsq;
+    a.mode = MODE_UNIFORM;
+    a.ua = int64_t(S.D) * S.D;
+    a.ub = S.D;
+    a.uc = 1;
+    a.ug = S.N;
+    a.n = B;
+    if (S.D == 1) {
+        a.tmpl = make_job(S, 0, KIND_LANE);
+        int grid = int((B + 63) / 64);
+        return launch_lds(a, 1, fused, grid, 64, 0, s);
+    }
+    int T = 64;
+    const int maxt = lds_maxt(S.D, fused);
+    int top = choose_top(S, fused, ds->max_lds, maxt, &T);
+    if (top == 0) {
+        a.tmpl = make_job(S, 0, KIND_BLOCK);
+        int64_t shmem = lds_bytes(S, 0, fused, T / MM_WAVE);
+        int grid = int(std::min<int64_t>(B, int64_t(1) << 30));
+        return launch_lds(a, S.D, fused, grid, T, shmem, s);
+    }
+    // Large lattice: LDS sweep of the sub-lattice {k_<top = 0}, then one launch
+    // per slab for the levels above, then the VJP reduction.
+    top = choose_top(S, false, ds->max_lds, lds_maxt(S.D, false), &T);
+    if (fused && (!ws || ws_bytes < vjp_ws_bytes(S, B)))
+        return fail(MM_ERR_WORKSPACE, "fused workspace of %zu bytes needed, %zu given", vjp_ws_bytes(S, B), ws_bytes);
+    for (int64_t t = 0; t < B; ++t) {
+        LdsArgs at = a;
+        at.P.g = nullptr;
+        at.n = 1;
+        at.tmpl = make_job(S, top, KIND_BLOCK);
+        at.tmpl.a_off = t * a.ua;
+        at.tmpl.b_off = t * a.ub;
+        at.tmpl.c_off = t;
+        at.tmpl.g_off = t * a.ug;
+        int64_t shmem = lds_bytes(S, top, false, T / MM_WAVE);
+        int rc = launch_lds(at, S.D, false, 1, T, shmem, s);
+        if (rc) return rc;
+        SlabArgs sa;
+        sa.g = geom_of(S);
+        sa.A = reinterpret_cast<const double2 *>(A) + t * a.ua;
+        sa.b = reinterpret_cast<const double2 *>(b) + t * a.ub;
+        sa.G = reinterpret_cast<double2 *>(G) + t * a.ug;
+        sa.sq = ds->sq;
+        sa.rsq = ds->rsq;
+        for (int d = top - 1; d >= 0; --d) {
+            for (int m = 1; m < S.s[d]; ++m) {
+                sa.d = d;
+                sa.m = m;
+                rc = launch_slab(sa, S.D, S.st[d], ds->num_cu, s);
+                if (rc) return rc;
+            }
+        }
+    }
+    if (fused) return launch_vjp(S, B, G, g, c, dA, db, dc, ws, ws_bytes, ds, s);
+    return MM_OK;
+}
+
+// ------------------------------------------------------------------- plans
+
+struct GlobalJob {
+    Shape S;
+    int64_t a_off, b_off, c_off, g_off;
+};
+
+}  // namespace
+
+struct mm_plan {
+    int device = 0;
+    bool fused = false;
+    int dmax = 3;
+    int threads = 256;
+    int grid = 1;
+    int64_t shmem = 0;
+    int wave_lds = 0;
+    int64_t nb = 0, nw = 0, nl = 0, n = 0;
+    Job *d_jobs = nullptr;
+    int *d_counters = nullptr;
+    double *d_partial = nullptr;
+    size_t partial_bytes = 0;
+    std::vector<GlobalJob> global;
+    int64_t n_amps = 0, a_elems = 0, b_elems = 0;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *gptr[9] = {nullptr};
+};
+
+namespace {
+
+int plan_launch(const mm_plan *p, const double *A, const double *b, const double *c, const double *g, double *G, double *dA,
+                double *db, double *dc, hipStream_t s) {
+    DevState *ds = nullptr;
+    int rc = device_state(&ds);
+    if (rc) return rc;
+    if (p->nb + p->nw + p->nl > 0) {
+        hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, p->d_counters, 3);
+        LAUNCH_CHECK("zero_ints_kernel");
+        LdsArgs a;
+        memset(&a, 0, sizeof(a));
+        a.P = bases(A, b, c, g, G, dA, db, dc, nullptr);
+        a.T.sq = ds->sq;
+        a.T.rsq = ds->rsq;
+        a.mode = MODE_QUEUE;
+        a.wave_lds = p->wave_lds;
+        a.table = p->d_jobs;
+        a.nb = p->nb;
+        a.nw = p->nw;
+        a.nl = p->nl;
+        a.counters = p->d_counters;
+        rc = launch_lds(a, p->dmax, p->fused, p->grid, p->threads, p->shmem, s);
+        if (rc) return rc;
+    }
+    for (const GlobalJob &gj : p->global) {
+        rc = vanilla_uniform(gj.S, 1, A + 2 * gj.a_off, b + 2 * gj.b_off, c + 2 * gj.c_off, g ? g + 2 * gj.g_off : nullptr,
+                             G + 2 * gj.g_off, dA ? dA + 2 * gj.a_off : nullptr, db ? db + 2 * gj.b_off : nullptr,
+                             dc ? dc + 2 * gj.c_off : nullptr, p->d_partial, p->partial_bytes, ds, s);
+        if (rc) return rc;
+    }
+    return MM_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char *mm_last_error(void) { return g_err.c_str(); }
+
+const char *mm_version(void) { return "mmhip 0.1 (gfx950)"; }
+
+int mm_init(int device) {
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (device != cur) HIP_TRY(hipSetDevice(device));
+    DevState *ds = nullptr;
+    int rc = device_state(&ds);
+    if (device != cur) HIP_TRY(hipSetDevice(cur));
+    return rc;
+}
+
+size_t mm_hermite_fwd_workspace(const int64_t *shape, int D, int stable) {
+    Shape S;
+    if (make_shape(shape, D, &S)) return 0;
+    return stable ? stable_ws_bytes(S) : 0;
+}
+
+int mm_hermite_fwd(const double *A, const double *b, const double *c, const int64_t *shape, int D, int stable, double *G,
+                   void *workspace, size_t ws_bytes, void *stream) {
+    Shape S;
+    int rc = make_shape(shape, D, &S);
+    if (rc) return rc;
+    if (!A || !b || !c || !G) return fail(MM_ERR_INVALID, "NULL array pointer");
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (stable) return launch_stable(S, 1, A, b, c, G, workspace, ws_bytes, ds, s);
+    return vanilla_uniform(S, 1, A, b, c, nullptr, G, nullptr, nullptr, nullptr, workspace, ws_bytes, ds, s);
+}
+
+size_t mm_hermite_batch_fwd_workspace(int64_t B, const int64_t *shape, int D, int stable) {
+    (void)B;
+    return mm_hermite_fwd_workspace(shape, D, stable);
+}
+
+int mm_hermite_batch_fwd(const double *A, const double *b, const double *c, int64_t B, const int64_t *shape, int D, int stable,
+                         double *G, void *workspace, size_t ws_bytes, void *stream) {
+    Shape S;
+    int rc = make_shape(shape, D, &S);
+    if (rc) return rc;
+    if (B < 0) return fail(MM_ERR_INVALID, "negative batch size");
+    if (B == 0) return MM_OK;
+    if (!A || !b || !c || !G) return fail(MM_ERR_INVALID, "NULL array pointer");
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (stable) return launch_stable(S, B, A, b, c, G, workspace, ws_bytes, ds, s);
+    return vanilla_uniform(S, B, A, b, c, nullptr, G, nullptr, nullptr, nullptr, workspace, ws_bytes, ds, s);
+}
+
+size_t mm_hermite_vjp_workspace(const int64_t *shape, int D) {
+    Shape S;
+    if (make_shape(shape, D, &S)) return 0;
+    return vjp_ws_bytes(S, 1);
+}
+
+int mm_hermite_vjp(const double *G, const double *c, const double *dLdG, const int64_t *shape, int D, double *dA, double *db,
+                   double *dc, void *workspace, size_t ws_bytes, void *stream) {
+    return mm_hermite_batch_vjp(G, c, dLdG, 1, shape, D, dA, db, dc, workspace, ws_bytes, stream);
+}
+
+size_t mm_hermite_batch_vjp_workspace(int64_t B, const int64_t *shape, int D) {
+    Shape S;
+    if (make_shape(shape, D, &S) || B < 1) return 0;
+    return vjp_ws_bytes(S, B);
+}
+
+int mm_hermite_batch_vjp(const double *G, const double *c, const double *dLdG, int64_t B, const int64_t *shape, int D, double *dA,
+                         double *db, double *dc, void *workspace, size_t ws_bytes, void *stream) {
+    Shape S;
+    int rc = make_shape(shape, D, &S);
+    if (rc) return rc;
+    if (B < 0) return fail(MM_ERR_INVALID, "negative batch size");
+    if (B == 0) return MM_OK;
+    if (B > 65535) return fail(MM_ERR_UNSUPPORTED, "batch of %lld > 65535 triples", (long long)B);
+    if (!G || !c || !dLdG || !dA || !db || !dc) return fail(MM_ERR_INVALID, "NULL array pointer");
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    return launch_vjp(S, B, G, dLdG, c, dA, db, dc, workspace, ws_bytes, ds, reinterpret_cast<hipStream_t>(stream));
+}
+
+size_t mm_hermite_fused_workspace(const int64_t *shape, int D) {
+    Shape S;
+    if (make_shape(shape, D, &S)) return 0;
+    DevState *ds = nullptr;
+    if (device_state(&ds)) return 0;
+    return fwd_ws_bytes(S, true, ds);
+}
+
+int mm_hermite_fused(const double *A, const double *b, const double *c, const double *dLdG, const int64_t *shape, int D, double *G,
+                     double *dA, double *db, double *dc, void *workspace, size_t ws_bytes, void *stream) {
+    return mm_hermite_batch_fused(A, b, c, dLdG, 1, shape, D, G, dA, db, dc, workspace, ws_bytes, stream);
+}
+
+size_t mm_hermite_batch_fused_workspace(int64_t B, const int64_t *shape, int D) {
+    Shape S;
+    if (make_shape(shape, D, &S) || B < 1) return 0;
+    DevState *ds = nullptr;
+    if (device_state(&ds)) return 0;
+    return fwd_ws_bytes(S, true, ds) ? vjp_ws_bytes(S, B) : 0;
+}
+
+int mm_hermite_batch_fused(const double *A, const double *b, const double *c, const double *dLdG, int64_t B, const int64_t *shape,
+                           int D, double *G, double *dA, double *db, double *dc, void *workspace, size_t ws_bytes, void *stream) {
+    Shape S;
+    int rc = make_shape(shape, D, &S);
+    if (rc) return rc;
+    if (B < 0) return fail(MM_ERR_INVALID, "negative batch size");
+    if (B == 0) return MM_OK;
+    if (!A || !b || !c || !dLdG || !G || !dA || !db || !dc) return fail(MM_ERR_INVALID, "NULL array pointer");
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    return vanilla_uniform(S, B, A, b, c, dLdG, G, dA, db, dc, workspace, ws_bytes, ds, reinterpret_cast<hipStream_t>(stream));
+}
+
+int mm_plan_create(int64_t n, const int32_t *dims, const int64_t *shapes, int fused, const mm_tune *tune, mm_plan **out) {
+    if (!out) return fail(MM_ERR_INVALID, "plan output pointer is NULL");
+    *out = nullptr;
+    if (n < 0) return fail(MM_ERR_INVALID, "negative number of triples");
+    if (n > 0 && (!dims || !shapes)) return fail(MM_ERR_INVALID, "NULL dims/shapes");
+    DevState *ds = nullptr;
+    int rc = device_state(&ds);
+    if (rc) return rc;
+    std::vector<Shape> S(static_cast<size_t>(n), Shape());
+    int dmax = 1;
+    for (int64_t k = 0; k < n; ++k) {
+        rc = make_shape(shapes + 8 * k, dims[k], &S[size_t(k)]);
+        if (rc) return fail(rc, "triple %lld: %s", (long long)k, g_err.c_str());
+        dmax = std::max(dmax, int(dims[k]));
+    }
+    mm_tune tn;
+    memset(&tn, 0, sizeof(tn));
+    if (tune) tn = *tune;
+    const bool fz = fused != 0;
+    const int kd = lds_kernel_dmax8(dmax) ? 8 : 3;
+    const int maxt = lds_maxt(kd, fz);
+    int T = tn.block_threads > 0 ? tn.block_threads : 256;
+    if (T % MM_WAVE || T < MM_WAVE || T > maxt) return fail(MM_ERR_INVALID, "block_threads=%d not a multiple of 64 in [64, %d]", T, maxt);
+    const int wpb = T / MM_WAVE;
+    int wave_max = tn.wave_max_slab == 0 ? 64 : tn.wave_max_slab;
+
+    auto p = new mm_plan();
+    HIP_TRY(hipGetDevice(&p->device));
+    p->fused = fz;
+    p->dmax = kd;
+    p->threads = T;
+    p->n = n;
+    std::vector<Job> blk, wav, lan;
+    std::vector<int64_t> cblk, cwav, clan;
+    int64_t a_off = 0, b_off = 0, g_off = 0;
+    int64_t block_lds = 0, wave_lds = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        const Shape &Sk = S[size_t(k)];
+        const int D = Sk.D;
+        int64_t steps = 1;
+        for (int j = 0; j < D; ++j) steps += Sk.s[j] - 1;
+        const int64_t cost = Sk.N + 64 * steps;
+        auto with_offsets = [&](Job J) {
+            J.a_off = a_off;
+            J.b_off = b_off;
+            J.c_off = k;
+            J.g_off = g_off;
+            return J;
+        };
+        if (D == 1) {
+            lan.push_back(with_offsets(make_job(Sk, 0, KIND_LANE)));
+            clan.push_back(cost);
+        } else {
+            const int64_t S0 = Sk.st[0];
+            const int64_t wl = packed_bits_ok(Sk, 0) ? lds_bytes(Sk, 0, fz, 1) : INT64_MAX;
+            const int64_t bl = packed_bits_ok(Sk, 0) ? lds_bytes(Sk, 0, fz, wpb) : INT64_MAX;
+            if (wave_max > 0 && S0 <= wave_max && wl * wpb <= ds->max_lds) {
+                wav.push_back(with_offsets(make_job(Sk, 0, KIND_WAVE)));
+                cwav.push_back(cost);
+                wave_lds = std::max(wave_lds, wl);
+            } else if (bl <= ds->max_lds && S0 <= int64_t(T) * 64) {
+                blk.push_back(with_offsets(make_job(Sk, 0, KIND_BLOCK)));
+                cblk.push_back(cost);
+                block_lds = std::max(block_lds, bl);
+            } else {
+                GlobalJob gj;
+                gj.S = Sk;
+                gj.a_off = a_off;
+                gj.b_off = b_off;
+                gj.c_off = k;
+                gj.g_off = g_off;
+                p->global.push_back(gj);
+                if (fz) p->partial_bytes = std::max(p->partial_bytes, vjp_ws_bytes(Sk, 1));
+            }
+        }
+        a_off += int64_t(D) * D;
+        b_off += D;
+        g_off += Sk.N;
+    }
+    p->n_amps = g_off;
+    p->a_elems = a_off;
+    p->b_elems = b_off;
+    // largest first (longest-processing-time order) within each class
+    auto sort_by_cost = [](std::vector<Job> &v, std::vector<int64_t> &c) {
+        std::vector<size_t> idx(v.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return c[x] > c[y]; });
+        std::vector<Job> out(v.size());
+        for (size_t i = 0; i < idx.size(); ++i) out[i] = v[idx[i]];
+        v.swap(out);
+    };
+    sort_by_cost(blk, cblk);
+    sort_by_cost(wav, cwav);
+    sort_by_cost(lan, clan);
+    p->nb = int64_t(blk.size());
+    p->nw = int64_t(wav.size());
+    p->nl = int64_t(lan.size());
+    p->wave_lds = int(align16(wave_lds));
+    p->shmem = std::max<int64_t>(block_lds, p->nw ? int64_t(p->wave_lds) * wpb : 0);
+    int bpc = tn.blocks_per_cu > 0 ? tn.blocks_per_cu
+                                   : int(std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(4, 2048 / T),
+                                                                                 p->shmem ? ds->max_lds / p->shmem : 4)));
+    const int64_t needed = p->nb + (p->nw + wpb - 1) / wpb + (p->nl + T - 1) / T;
+    p->grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(ds->num_cu) * bpc, needed)));
+    std::vector<Job> all;
+    all.insert(all.end(), blk.begin(), blk.end());
+    all.insert(all.end(), wav.begin(), wav.end());
+    all.insert(all.end(), lan.begin(), lan.end());
+    if (!all.empty()) {
+        HIP_TRY(hipMalloc(&p->d_jobs, sizeof(Job) * all.size()));
+        HIP_TRY(hipMemcpy(p->d_jobs, all.data(), sizeof(Job) * all.size(), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMalloc(&p->d_counters, 64));
+    if (p->partial_bytes) HIP_TRY(hipMalloc(&p->d_partial, p->partial_bytes));
+    HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+    *out = p;
+    return MM_OK;
+}
+
+int mm_plan_destroy(mm_plan *p) {
+    if (!p) return MM_OK;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    if (p->d_jobs) (void)hipFree(p->d_jobs);
+    if (p->d_counters) (void)hipFree(p->d_counters);
+    if (p->d_partial) (void)hipFree(p->d_partial);
+    delete p;
+    return MM_OK;
+}
+
+int mm_plan_info(const mm_plan *p, int64_t *n_amplitudes, int64_t *a_elems, int64_t *b_elems) {
+    if (!p) return fail(MM_ERR_INVALID, "NULL plan");
+    if (n_amplitudes) *n_amplitudes = p->n_amps;
+    if (a_elems) *a_elems = p->a_elems;
+    if (b_elems) *b_elems = p->b_elems;
+    return MM_OK;
+}
+
+int mm_plan_describe(const mm_plan *p, int64_t *out /* 8 entries */) {
+    if (!p || !out) return fail(MM_ERR_INVALID, "NULL plan or output");
+    out[0] = p->nb;
+    out[1] = p->nw;
+    out[2] = p->nl;
+    out[3] = int64_t(p->global.size());
+    out[4] = p->grid;
+    out[5] = p->threads;
+    out[6] = p->shmem;
+    out[7] = p->wave_lds;
+    return MM_OK;
+}
+
+static int plan_args_ok(const mm_plan *p, const double *A, const double *b, const double *c, const double *g, double *G, double *dA,
+                        double *db, double *dc) {
+    if (!p) return fail(MM_ERR_INVALID, "NULL plan");
+    if (p->n == 0) return MM_OK;
+    if (!A || !b || !c || !G) return fail(MM_ERR_INVALID, "NULL array pointer");
+    if (p->fused && (!g || !dA || !db || !dc)) return fail(MM_ERR_INVALID, "fused plan needs dLdG, dA, db, dc");
+    if (!p->fused && (g || dA || db || dc)) return fail(MM_ERR_INVALID, "forward plan takes no dLdG/dA/db/dc");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != p->device) return fail(MM_ERR_INVALID, "plan was created on device %d, current device is %d", p->device, dev);
+    return MM_OK;
+}
+
+int mm_plan_run(const mm_plan *p, const double *A, const double *b, const double *c, const double *dLdG, double *G, double *dA,
+                double *db, double *dc, void *stream) {
+    int rc = plan_args_ok(p, A, b, c, dLdG, G, dA, db, dc);
+    if (rc || p->n == 0) return rc;
+    return plan_launch(p, A, b, c, dLdG, G, dA, db, dc, reinterpret_cast<hipStream_t>(stream));
+}
+
+int mm_plan_run_graph(mm_plan *p, const double *A, const double *b, const double *c, const double *dLdG, double *G, double *dA,
+                      double *db, double *dc, void *stream) {
+    int rc = plan_args_ok(p, A, b, c, dLdG, G, dA, db, dc);
+    if (rc || p->n == 0) return rc;
+    const void *ptrs[9] = {A, b, c, dLdG, G, dA, db, dc, nullptr};
+    bool same = p->exec != nullptr;
+    for (int i = 0; i < 9 && same; ++i) same = ptrs[i] == p->gptr[i];
+    if (!same) {
+        if (p->exec) (void)hipGraphExecDestroy(p->exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        p->exec = nullptr;
+        p->graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        rc = plan_launch(p, A, b, c, dLdG, G, dA, db, dc, p->cap_stream);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(p->cap_stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(MM_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        p->graph = graph;
+        HIP_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+        for (int i = 0; i < 9; ++i) p->gptr[i] = ptrs[i];
+    }
+    HIP_TRY(hipGraphLaunch(p->exec, reinterpret_cast<hipStream_t>(stream)));
+    return MM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+"""torch.autograd wiring of the Hermite forward and its closed-form VJP.
+
+Counterpart of the reference's ``jax.custom_vjp`` pair
+(math/jax_vjps/hermite.py:47-102 unbatched, :113-175 batched): the forward
+keeps (G, c) and the backward calls the VJP kernel, whatever ``stable`` was,
+exactly as ``hermite_renormalized_jax_bwd`` always calls ``vanilla_vjp_numba``.
+
+Convention.  The reference VJP is holomorphic: it returns sum_k g_k dG_k/dtheta
+for the cotangent g, unconjugated.  torch hands a backward function
+grad = dL/d(conj G) and expects dL/d(conj theta), which for a holomorphic map
+is conj( sum_k conj(grad_k) dG_k/dtheta ).  So the backward conjugates the
+incoming gradient, runs the reference VJP, and conjugates the result.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from ..lattice import strategies as st
+
+
+class HermiteRenormalized(torch.autograd.Function):
+    """G = hermite_renormalized(A, b, c, shape, stable) with a custom VJP."""
+
+    @staticmethod
+    def forward(ctx, A, b, c, shape, stable):
+        G = st.stable_hip(shape, A, b, c) if stable else st.vanilla_hip(shape, A, b, c)
+        ctx.save_for_backward(G, c)
+        ctx.a_shape, ctx.b_shape, ctx.c_shape = A.shape, b.shape, c.shape
+        return G
+
+    @staticmethod
+    def backward(ctx, grad):
+        G, c = ctx.saved_tensors
+        dA, db, dc = st.vanilla_vjp_hip(G, c, grad.conj())
+        return (
+            dA.conj().reshape(ctx.a_shape),
+            db.conj().reshape(ctx.b_shape),
+            dc.conj().reshape(ctx.c_shape),
+            None,
+            None,
+        )
+
+
+class HermiteRenormalizedBatched(torch.autograd.Function):
+    """Batched G[k] = hermite_renormalized(A[k], b[k], c[k], shape, stable)."""
+
+    @staticmethod
+    def forward(ctx, A, b, c, shape, stable):
+        G = st.vanilla_batch_hip(shape, A, b, c, stable=stable)
+        ctx.save_for_backward(G, c)
+        ctx.a_shape, ctx.b_shape, ctx.c_shape = A.shape, b.shape, c.shape
+        return G
+
+    @staticmethod
+    def backward(ctx, grad):
+        G, c = ctx.saved_tensors
+        dA, db, dc = st.vanilla_batch_vjp_hip(G, c, grad.conj())
+        return (
+            dA.conj().reshape(ctx.a_shape),
+            db.conj().reshape(ctx.b_shape),
+            dc.conj().reshape(ctx.c_shape),
+            None,
+            None,
+        )

@@ -1,0 +1,116 @@
+"""``BackendHip``: the backend object behind ``mrmustard_amd.math``.
+
+It plays the role of ``BackendNumpy`` (math/backend_numpy.py) for the
+Bargmann->Fock path: ``hermite_renormalized`` and
+``hermite_renormalized_batched`` have the signatures of
+backend_numpy.py:381-403, but run the HIP kernels on the current GPU and
+return torch tensors.  The few generic tensor ops that the manager's
+dispatch uses (``astensor``, ``reshape``, ``broadcast_to``, ...) are torch
+operations.
+
+``out`` semantics follow the numpy backend: the result is written into
+``out`` and ``out`` itself is returned (test_vanilla.py:185).  ``out`` may be
+a GPU complex128 tensor (written in place) or a numpy array (filled by a
+device-to-host copy).  When any of A, b, c requires grad, the call goes
+through the autograd Function and ``out`` is rejected, as the JAX backend
+rejects it (backend_jax.py:485-486).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ..lattice import strategies as st
+from .autograd import HermiteRenormalized, HermiteRenormalizedBatched
+
+
+def _needs_grad(*xs) -> bool:
+    return torch.is_grad_enabled() and any(isinstance(x, torch.Tensor) and x.requires_grad for x in xs)
+
+
+class BackendHip:
+    """Torch/ROCm backend whose Hermite methods call libmmhip.so."""
+
+    name = "hip"
+
+    int32 = torch.int32
+    int64 = torch.int64
+    float32 = torch.float32
+    float64 = torch.float64
+    complex64 = torch.complex64
+    complex128 = torch.complex128
+
+    def __repr__(self) -> str:
+        return "BackendHip()"
+
+    # ------------------------------------------------------------ generic ops
+    def astensor(self, array, dtype=None):
+        if isinstance(array, torch.Tensor):
+            return array if dtype is None else array.to(dtype)
+        arr = np.asarray(array)
+        t = torch.as_tensor(arr)
+        return t if dtype is None else t.to(dtype)
+
+    def asnumpy(self, tensor) -> np.ndarray:
+        if isinstance(tensor, torch.Tensor):
+            return tensor.detach().resolve_conj().cpu().numpy()
+        return np.asarray(tensor)
+
+    def reshape(self, array, shape):
+        if isinstance(array, torch.Tensor):
+            return array.reshape(tuple(shape))
+        return np.reshape(array, tuple(shape))
+
+    def broadcast_to(self, array, shape):
+        if isinstance(array, torch.Tensor):
+            return array.broadcast_to(tuple(shape))
+        return np.broadcast_to(array, tuple(shape))
+
+    def zeros(self, shape, dtype=torch.complex128):
+        return torch.zeros(tuple(shape), dtype=dtype)
+
+    def conj(self, array):
+        return array.conj() if isinstance(array, torch.Tensor) else np.conj(array)
+
+    def sum(self, array, axis=None):
+        if isinstance(array, torch.Tensor):
+            return array.sum() if axis is None else array.sum(dim=axis)
+        return np.sum(array, axis=axis)
+
+    def allclose(self, a, b, rtol=1e-5, atol=1e-8) -> bool:
+        return bool(np.allclose(self.asnumpy(a), self.asnumpy(b), rtol=rtol, atol=atol))
+
+    # ---------------------------------------------------- hermite_renormalized
+    @staticmethod
+    def _deliver(G: torch.Tensor, out):
+        """Write G into a numpy `out` and return `out` itself (numpy backend semantics)."""
+        if out is None or isinstance(out, torch.Tensor):
+            return G if out is None else out
+        host = G.detach().cpu().numpy()
+        if host.size != np.size(out):
+            raise ValueError(f"out of shape {np.shape(out)} cannot hold an array of shape {host.shape}")
+        np.copyto(out, host.reshape(np.shape(out)))
+        return out
+
+    def hermite_renormalized(self, A, b, c, shape: tuple[int], stable: bool = False, out=None):
+        """Unbatched path (backend_numpy.py:381-392 -> core.py:25 / :127)."""
+        if _needs_grad(A, b, c):
+            if out is not None:
+                raise ValueError("out is not supported when gradients are tracked")
+            A, b, c = st.as_device_c128(A), st.as_device_c128(b), st.as_device_c128(c)
+            return HermiteRenormalized.apply(A, b, c, tuple(int(s) for s in shape), bool(stable))
+        dev_out = out if isinstance(out, torch.Tensor) else None
+        G = (st.stable_hip if stable else st.vanilla_hip)(shape, A, b, c, out=dev_out)
+        return self._deliver(G, out)
+
+    def hermite_renormalized_batched(self, A, b, c, shape: tuple[int], stable: bool = False, out=None):
+        """Batched path (backend_numpy.py:394-403 -> batch.py:27)."""
+        if _needs_grad(A, b, c):
+            if out is not None:
+                raise ValueError("out is not supported when gradients are tracked")
+            A, b, c = st.as_device_c128(A), st.as_device_c128(b), st.as_device_c128(c)
+            return HermiteRenormalizedBatched.apply(A, b, c, tuple(int(s) for s in shape), bool(stable))
+        dev_out = out if isinstance(out, torch.Tensor) else None
+        G = st.vanilla_batch_hip(shape, A, b, c, stable=stable, out=dev_out)
+        return self._deliver(G, out)
