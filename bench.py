@@ -1,0 +1,257 @@
+"""Benchmark of the Bargmann->Fock path on one MI355X.
+
+Headline workload: 256 three-mode Gaussian kets (A = U diag(tanh r) U^T,
+r ~ U[0, 1.2], b ~ CN(0, 0.5^2)), shape (30, 30, 30) each (6.9 M complex128
+amplitudes), with a CN(0, 1) cotangent: batched fused forward + VJP returning
+G, dA, db, dc.  One step = one pass over the whole batch.  `value` is its
+throughput in amplitudes/s with inputs resident in HBM.
+
+Also reported (extra keys): the same batch forward only; the plain path (one
+triple per launch, 256 launches per step); the uneven 4,096-ket work-queue
+batch; a single 2.56 M-amplitude lossy density matrix; and the CPU oracle
+(the C restatement of vanilla_numba + vanilla_vjp_numba, OpenMP over
+triples) on the headline workload.
+
+Prints one JSON line.  `--gpus` above 1 exits: this build measures one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Fock amplitudes/s fwd & fwd+VJP, batched vs plain GPU, at rel err ≤1e-12"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def fwd_flops_box(shape) -> int:
+    """Exact forward flop count of one box: sum_i (s_i-1) prod_{j>i} s_j (10(D-i)+8)."""
+    D = len(shape)
+    tot = 0
+    for i in range(D):
+        tail = int(np.prod(shape[i + 1 :])) if i + 1 < D else 1
+        tot += (shape[i] - 1) * tail * (10 * (D - i) + 8)
+    return tot
+
+
+def vjp_flops_per_amp(D: int) -> float:
+    return 8 * (D * (D + 1) / 2 + D + 1) + 1.5 * D * D + 4.5 * D
+
+
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch recorded by the rocprofv3 PMC runs, if committed."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--quick", action="store_true", help="headline measurement only (for profiling)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument(
+        "--copy-probe",
+        action="store_true",
+        help="only run a plain 1 GiB device-to-device copy (calibrates the HBM byte counters)",
+    )
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from mrmustard_amd import workloads as W
+    from mrmustard_amd.lattice import strategies as st
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    if args.copy_probe:
+        n = 1 << 27  # 2^27 float64 = 1 GiB
+        x = torch.ones(n, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}))
+        return
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+
+    (A_np, b_np, c_np), shape, g_np = W.batched_kets_30x3()
+    B = A_np.shape[0]
+    D = len(shape)
+    N = int(np.prod(shape))
+    amps = B * N
+    A = torch.as_tensor(A_np, device=dev)
+    b = torch.as_tensor(b_np, device=dev)
+    c = torch.as_tensor(c_np, device=dev)
+    g = torch.as_tensor(g_np, device=dev)
+    G = torch.empty((B, *shape), dtype=torch.complex128, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def fused_step():
+        return st.vanilla_batch_fused_hip(shape, A, b, c, g, out=G)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / steps  # seconds per step
+
+    # ---- headline: batched fused forward + VJP
+    log("headline: batched fused fwd+VJP, 256 x (30,30,30)")
+    t_fused = timed(fused_step, args.steps, args.warmup)
+    value = amps / t_fused
+
+    # parity of the measured configuration against the oracle (sample of triples)
+    import oracle
+
+    Gd, dA, db, dc = fused_step()
+    torch.cuda.synchronize()
+    Gh, dAh, dbh, dch = (x.cpu().numpy() for x in (Gd, dA, db, dc))
+    worst_G, worst_grad = 0.0, 0.0
+    for k in range(0, B, 37):
+        ref = oracle.vanilla(shape, A_np[k], b_np[k], c_np[k])
+        worst_G = max(worst_G, float(np.max(np.abs(Gh[k] - ref)) / np.max(np.abs(ref))))
+        for x, r in zip((dAh[k], dbh[k], dch[k]), oracle.vanilla_vjp(ref, c_np[k], g_np[k], kahan=True)):
+            worst_grad = max(worst_grad, float(np.max(np.abs(x - r)) / np.max(np.abs(r))))
+
+    bytes_per_launch = 32.0 * amps
+    flops_per_launch = B * (fwd_flops_box(shape) + vjp_flops_per_amp(D) * N)
+    achieved_gbs = bytes_per_launch / t_fused / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved_gbs, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profiles("traffic_fused_batch_30x30x30.txt"),
+        "kernel": "lds_kernel<3, true, 512> (uniform mode, one workgroup per triple)",
+        "fp64_tflops": round(flops_per_launch / t_fused / 1e12, 2),
+        "fp64_frac": round(flops_per_launch / t_fused / 1e12 / FP64_PEAK_TFLOPS, 4),
+    }
+    extra = {}
+
+    if not args.quick:
+        # ---- forward only, same batch
+        log("batched forward only")
+        t_fwd = timed(lambda: st.vanilla_batch_hip(shape, A, b, c, out=G), args.steps, args.warmup)
+        extra["batched_fwd_amps_per_s"] = amps / t_fwd
+        # ---- plain: one triple per launch
+        log("plain: one triple per launch")
+        Gk = [G[k] for k in range(B)]
+
+        def plain_step():
+            for k in range(B):
+                st.vanilla_fused_hip(shape, A[k], b[k], c[k], g[k], out=Gk[k])
+
+        t_plain = timed(plain_step, max(2, args.steps // 50), 1)
+        extra["plain_fused_amps_per_s"] = amps / t_plain
+        extra["batched_vs_plain_fused"] = round(t_plain / t_fused, 2)
+
+        def plain_fwd_step():
+            for k in range(B):
+                st.vanilla_hip(shape, A[k], b[k], c[k], out=Gk[k])
+
+        t_plain_fwd = timed(plain_fwd_step, max(2, args.steps // 50), 1)
+        extra["plain_fwd_amps_per_s"] = amps / t_plain_fwd
+        extra["batched_vs_plain_fwd"] = round(t_plain_fwd / t_fwd, 2)
+        # ---- uneven work-queue batch (4,096 kets, m in 1..3, cutoffs 8..60)
+        log("uneven 4096-ket work-queue batch")
+        from mrmustard_amd.autotune import tuned_plan
+
+        triples, shapes = W.uneven_kets()
+        plan, tune_info = tuned_plan(shapes, fused=False)
+        Au, bu, cu = plan.pack(triples)
+        Gu = torch.empty(plan.n_amplitudes, dtype=torch.complex128, device=dev)
+        t_unev = timed(lambda: plan.run(Au, bu, cu, G=Gu, graph=True), max(5, args.steps // 10), 2)
+        extra["uneven_batch_fwd_amps_per_s"] = plan.n_amplitudes / t_unev
+        extra["uneven_batch_amplitudes"] = plan.n_amplitudes
+        extra["uneven_batch_tune"] = tune_info
+        # ---- plain large lattice: 2-mode lossy density matrix, shape (40,)^4
+        log("plain lossy DM (40,)^4")
+        (A1, b1, c1), shape1 = W.lossy_dm_40x4()
+        A1t, b1t, c1t = (torch.as_tensor(np.asarray(x), device=dev) for x in (A1, b1, c1))
+        G1 = torch.empty(shape1, dtype=torch.complex128, device=dev)
+        t_dm = timed(lambda: st.vanilla_hip(shape1, A1t, b1t, c1t, out=G1), 20, 2)
+        extra["dm_40x4_fwd_amps_per_s"] = G1.numel() / t_dm
+
+    cpu = None
+    if not args.no_cpu and not args.quick:
+        log("CPU oracle baseline")
+        threads = oracle.max_threads()
+        triples = [(A_np[k], b_np[k], c_np[k]) for k in range(B)]
+        shapes = [shape] * B
+        gs = [g_np[k] for k in range(B)]
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            oracle.uneven(triples, shapes, gs)
+            reps += 1
+            if time.perf_counter() - t0 > 5.0:
+                break
+        wall = time.perf_counter() - t0
+        cpu = {
+            "value": reps * amps / wall,
+            "unit": "amplitudes/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"the full headline batch (256 x (30,30,30), forward + VJP) x {reps} repetitions, "
+            f"{wall:.1f} s wall, OpenMP over triples",
+        }
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "amplitudes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_fused * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched fused forward+VJP: 256 three-mode Gaussian kets, shape (30,30,30), complex128",
+            "triples": B,
+            "shape": list(shape),
+            "amplitudes_per_step": amps,
+            "parallelism": "single GPU",
+        },
+        "parity": {"max_rel_err_G": worst_G, "max_rel_err_grads": worst_grad, "checked_triples": len(range(0, B, 37))},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
