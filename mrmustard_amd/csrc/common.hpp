@@ -55,7 +55,7 @@ __device__ __forceinline__ double div_by(double x, double s, double r) {
     double q = x * r;
     double e = fma(-q, s, x);
     double qq = fma(e, r, q);
-    if (__builtin_expect(fabs(x) < 0x1p-960, 0)) qq = x / s;
+    if (__builtin_expect((fabs(x) < 0x1p-960) & (x != 0.0), 0)) qq = x / s;
     return qq;
 }
 
@@ -103,6 +103,79 @@ __device__ __forceinline__ double2 wave_sum(double2 v) {
         v.y += shfl_xor_d(v.y, m);
     }
     return v;
+}
+
+// DPP move of a double (two 32-bit halves); rows outside ROW_MASK read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sums of N values over the wave with DPP lane moves (no LDS traffic):
+// pairs, quads, half rows, rows, then rows 0+1 and all four rows through the
+// row broadcasts.  Lane 63 holds the totals; the order is fixed, so the
+// result is deterministic.
+template <int N>
+__device__ __forceinline__ void wave_sum_dpp(double2 (&v)[N]) {
+#define MM_DPP_STEP(CTRL, MASK)                          \
+    _Pragma("unroll") for (int e = 0; e < N; ++e) {      \
+        v[e].x += dpp_d<CTRL, MASK>(v[e].x);             \
+        v[e].y += dpp_d<CTRL, MASK>(v[e].y);             \
+    }
+    MM_DPP_STEP(0xB1, 0xF)   // quad_perm [1,0,3,2]
+    MM_DPP_STEP(0x4E, 0xF)   // quad_perm [2,3,0,1]
+    MM_DPP_STEP(0x141, 0xF)  // row_half_mirror
+    MM_DPP_STEP(0x140, 0xF)  // row_mirror
+    MM_DPP_STEP(0x142, 0xA)  // row_bcast:15 into rows 1 and 3
+    MM_DPP_STEP(0x143, 0xC)  // row_bcast:31 into rows 2 and 3
+#undef MM_DPP_STEP
+}
+
+// Butterfly sums of N values over the wave, stage by stage so that the N
+// shuffles of a stage are in flight together.  Lane 0 holds the totals.
+template <int N>
+__device__ __forceinline__ void wave_sum_many(double2 (&v)[N]) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        double tx[N], ty[N];
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+            tx[e] = shfl_xor_d(v[e].x, m);
+            ty[e] = shfl_xor_d(v[e].y, m);
+        }
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+            v[e].x += tx[e];
+            v[e].y += ty[e];
+        }
+    }
+}
+
+// cdiv_by on N values with a single branch for the rare tiny numerators.
+template <int N>
+__device__ __forceinline__ void cdiv_many(double2 (&v)[N], double s, double r) {
+    double2 out[N];
+    bool tiny = false;
+#pragma unroll
+    for (int e = 0; e < N; ++e) {
+        double qx = v[e].x * r, qy = v[e].y * r;
+        const double ex = fma(-qx, s, v[e].x), ey = fma(-qy, s, v[e].y);
+        out[e].x = fma(ex, r, qx);
+        out[e].y = fma(ey, r, qy);
+        // exact zeros come out right (up to the sign of zero) and need no fix-up
+        tiny |= ((fabs(v[e].x) < 0x1p-960) & (v[e].x != 0.0)) | ((fabs(v[e].y) < 0x1p-960) & (v[e].y != 0.0));
+    }
+    if (__builtin_expect(tiny, 0)) {
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+            if (fabs(v[e].x) < 0x1p-960) out[e].x = v[e].x / s;
+            if (fabs(v[e].y) < 0x1p-960) out[e].y = v[e].y / s;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < N; ++e) v[e] = out[e];
 }
 
 }  // namespace mm

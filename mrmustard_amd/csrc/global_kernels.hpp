@@ -137,8 +137,7 @@ __global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
         }
     }
     __shared__ double2 red[(256 / MM_WAVE) * NA];
-#pragma unroll
-    for (int e = 0; e < NA; ++e) acc[e] = wave_sum(acc[e]);
+    wave_sum_many<NA>(acc);
     const int lane = threadIdx.x & (MM_WAVE - 1), w = threadIdx.x / MM_WAVE;
     if (lane == 0) {
 #pragma unroll

@@ -154,10 +154,16 @@ Job make_job(const Shape &S, int top, int kind) {
     return J;
 }
 
-int64_t lds_bytes(const Shape &S, int top, bool fused, int nwaves) {
+bool lds_kernel_dmax8(int D) { return D > 3; }
+
+int kernel_dmax(int D) { return lds_kernel_dmax8(D) ? 8 : 3; }
+
+// LDS bytes of one team sweeping S from level `top` in the kernel
+// instantiation for `kdmax` (3 or 8) indices.
+int64_t lds_bytes(const Shape &S, int top, bool fused, int nwaves, int kdmax) {
     int smax = 1;
     for (int j = top; j < S.D; ++j) smax = std::max(smax, int(S.s[j]));
-    return lds_layout(S.D, top, int(S.st[top]), smax, fused ? 1 : 0, nwaves).total;
+    return lds_layout(S.D, top, int(S.st[top]), smax, fused ? 1 : 0, nwaves, rmax_for(kdmax, fused)).total;
 }
 
 bool packed_bits_ok(const Shape &S, int top) {
@@ -176,16 +182,39 @@ int team_threads(int64_t slab, int maxt) {
     return int(t);
 }
 
-bool lds_kernel_dmax8(int D) { return D > 3; }
-int lds_maxt(int D, bool fused) { return fused ? (lds_kernel_dmax8(D) ? 256 : 512) : 1024; }
+int lds_maxt(int D, bool fused) { return fused ? (lds_kernel_dmax8(D) ? 256 : 512) : (lds_kernel_dmax8(D) ? 512 : 768); }
+
+// Threads for a team sweeping slabs of S points with at most `rmax` points
+// per thread in registers: fewest thread-slots T * ceil(S/T), counting a
+// team of fewer than 8 waves (2 per SIMD) as proportionally slower, ties to
+// more threads.  0 if no T <= maxt holds the slab in registers.
+int pick_threads(int64_t S, int maxt, int rmax) {
+    int best = 0;
+    double best_cost = 0;
+    for (int T = MM_WAVE; T <= maxt; T += MM_WAVE) {
+        const int64_t nr = (S + T - 1) / T;
+        if (nr > rmax) continue;
+        const double waves = T / MM_WAVE;
+        const double cost = double(T * nr) * (waves < 8 ? 8.0 / waves : 1.0);
+        if (best == 0 || cost < best_cost || (cost == best_cost && T > best)) {
+            best = T;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
 
 // Smallest top level whose sweep fits in LDS with `threads` threads (D >= 2).
 int choose_top(const Shape &S, bool fused, int max_lds, int maxt, int *threads) {
     for (int top = 0; top < S.D; ++top) {
         int T = team_threads(S.st[top], maxt);
+        if (top == 0) {
+            const int Tr = pick_threads(S.st[0], maxt, rmax_for(kernel_dmax(S.D), fused));
+            if (Tr) T = Tr;
+        }
         if (S.st[top] > int64_t(8) * 1024 * 1024) continue;
         if (!packed_bits_ok(S, top)) continue;
-        if (lds_bytes(S, top, fused, T / MM_WAVE) <= max_lds) {
+        if (lds_bytes(S, top, fused, T / MM_WAVE, kernel_dmax(S.D)) <= max_lds) {
             *threads = T;
             return top;
         }
@@ -206,16 +235,16 @@ int launch_lds(const LdsArgs &a, int D, bool fused, int grid, int threads, int64
             raise_lds_limit(lds_kernel<8, true, 256>, int(shmem));
             hipLaunchKernelGGL((lds_kernel<8, true, 256>), dim3(grid), dim3(threads), size_t(shmem), s, a);
         } else {
-            raise_lds_limit(lds_kernel<8, false, 1024>, int(shmem));
-            hipLaunchKernelGGL((lds_kernel<8, false, 1024>), dim3(grid), dim3(threads), size_t(shmem), s, a);
+            raise_lds_limit(lds_kernel<8, false, 512>, int(shmem));
+            hipLaunchKernelGGL((lds_kernel<8, false, 512>), dim3(grid), dim3(threads), size_t(shmem), s, a);
         }
     } else {
         if (fused) {
             raise_lds_limit(lds_kernel<3, true, 512>, int(shmem));
             hipLaunchKernelGGL((lds_kernel<3, true, 512>), dim3(grid), dim3(threads), size_t(shmem), s, a);
         } else {
-            raise_lds_limit(lds_kernel<3, false, 1024>, int(shmem));
-            hipLaunchKernelGGL((lds_kernel<3, false, 1024>), dim3(grid), dim3(threads), size_t(shmem), s, a);
+            raise_lds_limit(lds_kernel<3, false, 768>, int(shmem));
+            hipLaunchKernelGGL((lds_kernel<3, false, 768>), dim3(grid), dim3(threads), size_t(shmem), s, a);
         }
     }
     LAUNCH_CHECK("lds_kernel");
@@ -398,7 +427,7 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
     int top = choose_top(S, fused, ds->max_lds, maxt, &T);
     if (top == 0) {
         a.tmpl = make_job(S, 0, KIND_BLOCK);
-        int64_t shmem = lds_bytes(S, 0, fused, T / MM_WAVE);
+        int64_t shmem = lds_bytes(S, 0, fused, T / MM_WAVE, kernel_dmax(S.D));
         int grid = int(std::min<int64_t>(B, int64_t(1) << 30));
         return launch_lds(a, S.D, fused, grid, T, shmem, s);
     }
@@ -416,7 +445,7 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
         at.tmpl.b_off = t * a.ub;
         at.tmpl.c_off = t;
         at.tmpl.g_off = t * a.ug;
-        int64_t shmem = lds_bytes(S, top, false, T / MM_WAVE);
+        int64_t shmem = lds_bytes(S, top, false, T / MM_WAVE, kernel_dmax(S.D));
         int rc = launch_lds(at, S.D, false, 1, T, shmem, s);
         if (rc) return rc;
         SlabArgs sa;
@@ -685,8 +714,8 @@ int mm_plan_create(int64_t n, const int32_t *dims, const int64_t *shapes, int fu
             clan.push_back(cost);
         } else {
             const int64_t S0 = Sk.st[0];
-            const int64_t wl = packed_bits_ok(Sk, 0) ? lds_bytes(Sk, 0, fz, 1) : INT64_MAX;
-            const int64_t bl = packed_bits_ok(Sk, 0) ? lds_bytes(Sk, 0, fz, wpb) : INT64_MAX;
+            const int64_t wl = packed_bits_ok(Sk, 0) ? lds_bytes(Sk, 0, fz, 1, kd) : INT64_MAX;
+            const int64_t bl = packed_bits_ok(Sk, 0) ? lds_bytes(Sk, 0, fz, wpb, kd) : INT64_MAX;
             if (wave_max > 0 && S0 <= wave_max && wl * wpb <= ds->max_lds) {
                 wav.push_back(with_offsets(make_job(Sk, 0, KIND_WAVE)));
                 cwav.push_back(cost);

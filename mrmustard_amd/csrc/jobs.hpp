@@ -47,15 +47,24 @@ __host__ __device__ inline int64_t align16(int64_t x) { return (x + 15) & ~int64
 
 // Byte offsets of the LDS regions of one team.
 struct LdsLayout {
-    int64_t job, ring, kv, coef, sq, rsq, bvec, red, total;
+    int64_t job, ring, kv, coef, sq, rsq, bvec, amat, red, total;
 };
 
 // Bytes reserved at the start of a team's LDS region for its Job descriptor.
 constexpr int64_t JOB_LDS_BYTES = 256;
 
-__host__ __device__ inline LdsLayout lds_layout(int D, int top, int S, int smax, int fused, int nwaves) {
+// Largest number of level-`top` slab points one thread keeps in registers,
+// per kernel instantiation (kernels for D <= 3, or for D <= 8).
+__host__ __device__ constexpr int rmax_for(int dmax, bool fused) { return dmax <= 3 ? (fused ? 2 : 4) : (fused ? 1 : 2); }
+
+// The register path (whole triples only, top = 0) holds every level-0 slab
+// point of a team in registers; it needs two LDS slab buffers even when
+// fused.  Otherwise the fused sweep keeps three.
+__host__ __device__ inline bool reg_path(int S, int nwaves, int rmax) { return int64_t(S) <= int64_t(rmax) * nwaves * 64; }
+
+__host__ __device__ inline LdsLayout lds_layout(int D, int top, int S, int smax, int fused, int nwaves, int rmax) {
     LdsLayout L;
-    int nr = fused ? 3 : 2;
+    int nr = (fused && !(top == 0 && reg_path(S, nwaves, rmax))) ? 3 : 2;
     int64_t off = 0;
     L.job = off;
     off += JOB_LDS_BYTES;
@@ -71,6 +80,8 @@ __host__ __device__ inline LdsLayout lds_layout(int D, int top, int S, int smax,
     off = align16(off + int64_t(smax) * 8);
     L.bvec = off;
     off = align16(off + int64_t(D + 1) * 16);
+    L.amat = off;
+    off = align16(off + int64_t(D) * D * 16);
     L.red = off;
     off = align16(off + (fused ? int64_t(nwaves) * nacc_of(D) * 16 : 0));
     L.total = off;

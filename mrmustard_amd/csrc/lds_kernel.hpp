@@ -26,51 +26,35 @@ struct LdsArgs {
 };
 
 template <int DMAX, bool FUSED, bool WAVE>
-__device__ __forceinline__ void run_team(const Job *J, const JobShift &sh, const LdsArgs &a, char *lds) {
-    switch (J->D) {
+__device__ __forceinline__ void run_team(const Job *src, const JobShift &sh, const Bases &P, const Tables &T, char *lds) {
+    load_job<WAVE>(lds, src, sh);
+    const int D = reinterpret_cast<const Job *>(lds)->D;
+    constexpr int RM = rmax_for(DMAX, FUSED);
+    switch (D) {
         case 2:
-            if constexpr (DMAX >= 2) sweep_lds<2, FUSED, WAVE>(J, sh, a.P, a.T, lds);
+            if constexpr (DMAX >= 2) sweep_lds<2, FUSED, WAVE, RM>(P, T, lds);
             break;
         case 3:
-            if constexpr (DMAX >= 3) sweep_lds<3, FUSED, WAVE>(J, sh, a.P, a.T, lds);
+            if constexpr (DMAX >= 3) sweep_lds<3, FUSED, WAVE, RM>(P, T, lds);
             break;
         case 4:
-            if constexpr (DMAX >= 4) sweep_lds<4, FUSED, WAVE>(J, sh, a.P, a.T, lds);
+            if constexpr (DMAX >= 4) sweep_lds<4, FUSED, WAVE, RM>(P, T, lds);
             break;
         case 5:
-            if constexpr (DMAX >= 5) sweep_lds<5, FUSED, WAVE>(J, sh, a.P, a.T, lds);
+            if constexpr (DMAX >= 5) sweep_lds<5, FUSED, WAVE, RM>(P, T, lds);
             break;
         case 6:
-            if constexpr (DMAX >= 6) sweep_lds<6, FUSED, WAVE>(J, sh, a.P, a.T, lds);
+            if constexpr (DMAX >= 6) sweep_lds<6, FUSED, WAVE, RM>(P, T, lds);
             break;
         case 7:
-            if constexpr (DMAX >= 7) sweep_lds<7, FUSED, WAVE>(J, sh, a.P, a.T, lds);
+            if constexpr (DMAX >= 7) sweep_lds<7, FUSED, WAVE, RM>(P, T, lds);
             break;
         case 8:
-            if constexpr (DMAX >= 8) sweep_lds<8, FUSED, WAVE>(J, sh, a.P, a.T, lds);
+            if constexpr (DMAX >= 8) sweep_lds<8, FUSED, WAVE, RM>(P, T, lds);
             break;
         default:
             break;
     }
-}
-
-__device__ __forceinline__ JobShift shift_of(const LdsArgs &a, int64_t k) {
-    JobShift sh;
-    sh.k = k;
-    sh.ua = a.ua;
-    sh.ub = a.ub;
-    sh.uc = a.uc;
-    sh.ug = a.ug;
-    return sh;
-}
-
-__device__ __forceinline__ Job shifted(const LdsArgs &a, int64_t k) {
-    Job J = a.tmpl;
-    J.a_off += k * a.ua;
-    J.b_off += k * a.ub;
-    J.c_off += k * a.uc;
-    J.g_off += k * a.ug;
-    return J;
 }
 
 template <int DMAX, bool FUSED, int MAXT>
@@ -79,24 +63,38 @@ __global__ __launch_bounds__(MAXT) void lds_kernel(LdsArgs a) {
     const int lane = threadIdx.x & (MM_WAVE - 1);
     const int wave = threadIdx.x / MM_WAVE;
     const int wpb = blockDim.x / MM_WAVE;
+    const Bases P = a.P;
+    const Tables T = a.T;
 
     if (a.mode == MODE_UNIFORM) {
+        JobShift sh;
+        sh.ua = a.ua;
+        sh.ub = a.ub;
+        sh.uc = a.uc;
+        sh.ug = a.ug;
         const int kind = a.tmpl.kind;
         if (kind == KIND_BLOCK) {
-            for (int64_t k = blockIdx.x; k < a.n; k += gridDim.x) run_team<DMAX, FUSED, false>(&a.tmpl, shift_of(a, k), a, lds);
-        } else if (kind == KIND_WAVE) {
-            for (int64_t k = int64_t(blockIdx.x) * wpb + wave; k < a.n; k += int64_t(gridDim.x) * wpb)
-                run_team<DMAX, FUSED, true>(&a.tmpl, shift_of(a, k), a, lds + wave * a.wave_lds);
-        } else {
-            for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < a.n; k += int64_t(gridDim.x) * blockDim.x) {
-                const Job J = shifted(a, k);
-                chain_lane<FUSED>(J, a.P, a.T);
+            for (int64_t k = blockIdx.x; k < a.n; k += gridDim.x) {
+                sh.k = k;
+                run_team<DMAX, FUSED, false>(&a.tmpl, sh, P, T, lds);
             }
+        } else if (kind == KIND_WAVE) {
+            for (int64_t k = int64_t(blockIdx.x) * wpb + wave; k < a.n; k += int64_t(gridDim.x) * wpb) {
+                sh.k = k;
+                run_team<DMAX, FUSED, true>(&a.tmpl, sh, P, T, lds + wave * a.wave_lds);
+            }
+        } else {
+            const int s0 = a.tmpl.shape[0];
+            for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < a.n; k += int64_t(gridDim.x) * blockDim.x)
+                chain_lane<FUSED>(s0, a.tmpl.a_off + k * a.ua, a.tmpl.b_off + k * a.ub, a.tmpl.c_off + k * a.uc, a.tmpl.g_off + k * a.ug, P, T);
         }
         return;
     }
 
     // ---- queue mode.  Every loop ends once its counter passes its job count.
+    JobShift none;
+    none.k = 0;
+    none.ua = none.ub = none.uc = none.ug = 0;
     __shared__ int s_idx;
     for (;;) {
         if (threadIdx.x == 0) s_idx = atomicAdd(&a.counters[0], 1);
@@ -104,7 +102,7 @@ __global__ __launch_bounds__(MAXT) void lds_kernel(LdsArgs a) {
         const int idx = s_idx;
         __syncthreads();
         if (idx >= a.nb) break;
-        run_team<DMAX, FUSED, false>(a.table + idx, shift_of(a, 0), a, lds);
+        run_team<DMAX, FUSED, false>(a.table + idx, none, P, T, lds);
     }
     char *wl = lds + wave * a.wave_lds;
     for (;;) {
@@ -112,7 +110,7 @@ __global__ __launch_bounds__(MAXT) void lds_kernel(LdsArgs a) {
         if (lane == 0) idx = atomicAdd(&a.counters[1], 1);
         idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0, MM_WAVE));
         if (idx >= a.nw) break;
-        run_team<DMAX, FUSED, true>(a.table + a.nb + idx, shift_of(a, 0), a, wl);
+        run_team<DMAX, FUSED, true>(a.table + a.nb + idx, none, P, T, wl);
     }
     for (;;) {
         int base = 0;
@@ -121,8 +119,8 @@ __global__ __launch_bounds__(MAXT) void lds_kernel(LdsArgs a) {
         if (base >= a.nl) break;
         const int64_t idx = int64_t(base) + lane;
         if (idx < a.nl) {
-            const Job J = a.table[a.nb + a.nw + idx];
-            chain_lane<FUSED>(J, a.P, a.T);
+            const Job *J = a.table + a.nb + a.nw + idx;
+            chain_lane<FUSED>(J->shape[0], J->a_off, J->b_off, J->c_off, J->g_off, P, T);
         }
     }
 }

@@ -21,6 +21,12 @@
 #include "common.hpp"
 #include "jobs.hpp"
 
+// Phase markers of the sweep (set up, face levels, face VJP, level 0,
+// reduction).  Empty in the library; a timing build may define them.
+#ifndef MM_PHASE
+#define MM_PHASE(i) ((void)0)
+#endif
+
 namespace mm {
 
 struct Bases {
@@ -75,10 +81,15 @@ struct Acc {
 
 __device__ __forceinline__ int digit(uint32_t kp, const Job &J, int j) { return int((kp >> J.kbit[j]) & uint32_t(J.kmask[j])); }
 
-// G*gk products accumulated with weight w:  acc += w * (Gv * gk)
+// acc += w * Gv * gk for real w: (w Gv) then a complex multiply-accumulate
+// in four fused multiply-adds.
 __device__ __forceinline__ double2 wacc(double2 acc, double w, double2 Gv, double2 gk) {
-    double2 p = make_double2(Gv.x * gk.x - Gv.y * gk.y, Gv.x * gk.y + Gv.y * gk.x);
-    return caxpy(w, p, acc);
+    const double ux = w * Gv.x, uy = w * Gv.y;
+    acc.x = fma(ux, gk.x, acc.x);
+    acc.x = fma(-uy, gk.y, acc.x);
+    acc.y = fma(ux, gk.y, acc.y);
+    acc.y = fma(uy, gk.x, acc.y);
+    return acc;
 }
 
 // VJP terms of one point whose axes < `lo` are zero and whose digits for
@@ -137,22 +148,28 @@ __device__ void reduce_and_store(Acc<D> &acc, const Job &J, const Bases &P, cons
 #pragma unroll
     for (int e = 0; e < D; ++e) v[D * (D + 1) / 2 + e] = acc.b[e];
     v[NA - 1] = acc.c;
-#pragma unroll
-    for (int e = 0; e < NA; ++e) v[e] = wave_sum(v[e]);
+    wave_sum_dpp<NA>(v);
+    const int lane = threadIdx.x & (MM_WAVE - 1);
     if (!WAVE) {
-        int lane = threadIdx.x & (MM_WAVE - 1);
-        if (lane == 0) {
+        // lane 63 of every wave publishes; thread e then sums accumulator e
+        // over the waves in wave order, and thread 0 gathers the totals
+        if (lane == MM_WAVE - 1) {
 #pragma unroll
             for (int e = 0; e < NA; ++e) red[tm.wave * NA + e] = v[e];
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int w = 1; w < tm.nwaves; ++w)
-#pragma unroll
-                for (int e = 0; e < NA; ++e) v[e] = cadd(v[e], red[w * NA + e]);
+        if (threadIdx.x < NA) {
+            double2 t = red[threadIdx.x];
+            for (int w = 1; w < tm.nwaves; ++w) t = cadd(t, red[w * NA + threadIdx.x]);
+            red[threadIdx.x] = t;
         }
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+#pragma unroll
+        for (int e = 0; e < NA; ++e) v[e] = red[e];
+    } else if (lane != MM_WAVE - 1) {
+        return;
     }
-    if (tm.rank != 0) return;
     if (J.flags & JOB_WRITE_PARTIAL) {
         double2 *out = P.partial + J.p_off;
 #pragma unroll
@@ -175,47 +192,390 @@ __device__ void reduce_and_store(Acc<D> &acc, const Job &J, const Bases &P, cons
     P.dc[J.c_off] = cdiv_py(v[NA - 1], c);
 }
 
+// Level 0 of a triple swept whole in LDS, with every slab point of the team
+// held in registers (RM points per thread): a point's own values of slabs
+// m-1 and m-2, its coefficients A_0j sqrt(k_j) and digits stay in registers
+// across steps, so LDS is read only for the neighbours k - 1_j.  Fused: the
+// cotangent of slab m+1 is loaded during step m, and the slab-m neighbours
+// read by the VJP of step m are kept for the forward of step m+1.  Two LDS
+// slab buffers suffice.
+//
+// The loop body is branch-free so that all loads of a step can be issued
+// together: a term whose digit is zero reads the point itself and is
+// multiplied by a zero coefficient, which is what the reference does too
+// (it adds A_ij * SQRT[0] * G[...], core.py:101-103); points past the slab
+// end compute on a clamped index, store nothing and carry a zero cotangent.
+template <int D, bool FUSED, bool WAVE, int RM>
+__device__ __forceinline__ void level0_reg(const Job &J, const Team<WAVE> &tm, double2 *ring, const uint32_t *kv,
+                                           const double *sq, const double *rsq, const double2 *bv, const double2 *A, double2 *G,
+                                           const double2 *g, const double2 *coef, Acc<D> &acc) {
+    constexpr bool CJ_REG = FUSED || (D - 1) * RM <= 8;
+    const int smax = J.smax;
+    const int S = J.S;
+    const int s0 = J.shape[0];
+    int st[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) st[j] = int(J.stride[j]);
+    // rounds of `tm.size` points actually needed (uniform across the team)
+    const int nr = (S + tm.size - 1) / tm.size;
+    int q[RM];
+    bool act[RM];
+    int kd[RM][D];
+    double sqk[RM][D];
+    double2 cj[RM][D];
+    double2 nb[RM][D];
+    double2 p1v[RM], p2v[RM], gcur[RM], gnext[RM];
+    const double2 b0 = bv[0];
+    const double2 a00 = A[0];
+    const double2 zero = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+        const int qq = tm.rank + r * tm.size;
+        act[r] = qq < S;
+        q[r] = act[r] ? qq : S - 1;
+        const uint32_t kp = kv[q[r]];
+#pragma unroll
+        for (int j = 1; j < D; ++j) {
+            kd[r][j] = digit(kp, J, j);
+            sqk[r][j] = sq[kd[r][j]];
+            cj[r][j] = CJ_REG ? cscale_promoted(A[j], sqk[r][j]) : zero;
+            nb[r][j] = zero;
+        }
+        p1v[r] = ring[q[r]];
+        p2v[r] = zero;
+        gcur[r] = zero;
+        if (FUSED && s0 > 1) {
+            const double2 t = g[int64_t(S) + q[r]];
+            gcur[r] = act[r] ? t : zero;
+        }
+        gnext[r] = zero;
+    }
+    for (int m = 1; m < s0; ++m) {
+        double2 *cur = ring + (m & 1) * S;
+        const double2 *p1 = ring + ((m - 1) & 1) * S;
+        const double sm = sq[m], rm = rsq[m];
+        const double sm1 = sq[m - 1];
+        const double2 c00 = cscale_promoted(a00, sm1);
+        const int64_t gbase = int64_t(m) * S;
+        if (FUSED && m + 1 < s0) {
+#pragma unroll
+            for (int r = 0; r < RM; ++r) {
+                if (r >= nr) break;
+                const double2 t = g[gbase + S + q[r]];
+                gnext[r] = act[r] ? t : zero;
+            }
+        }
+        double2 vnew[RM];
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+            vnew[r] = zero;
+            if (r >= nr) break;
+            double2 v = cmul(b0, p1v[r]);
+            v = cadd(v, cmul(c00, p2v[r]));
+#pragma unroll
+            for (int j = 1; j < D; ++j) {
+                if (!FUSED || m == 1) nb[r][j] = p1[q[r] - (kd[r][j] > 0 ? st[j] : 0)];
+                const double2 cf = CJ_REG ? cj[r][j] : coef[coef_row(D, 0, 0, j) * smax + kd[r][j]];
+                v = cadd(v, cmul(cf, nb[r][j]));
+            }
+            vnew[r] = cdiv_by(v, sm, rm);
+        }
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+            if (r >= nr) break;
+            if (act[r]) {
+                cur[q[r]] = vnew[r];
+                G[gbase + q[r]] = vnew[r];
+            }
+        }
+        tm.sync();
+        if (FUSED) {
+            const double w00 = 0.5 * sm * sm1;
+#pragma unroll
+            for (int r = 0; r < RM; ++r) {
+                if (r >= nr) break;
+                const double2 gk = gcur[r];
+                acc.c = cfma(vnew[r], gk, acc.c);
+                acc.b[0] = wacc(acc.b[0], sm, p1v[r], gk);
+                acc.a[0] = wacc(acc.a[0], w00, p2v[r], gk);
+#pragma unroll
+                for (int j = 1; j < D; ++j) acc.a[Acc<D>::tri(0, j)] = wacc(acc.a[Acc<D>::tri(0, j)], sm * sqk[r][j], nb[r][j], gk);
+#pragma unroll
+                for (int i = 1; i < D; ++i) {
+                    const int ki = kd[r][i];
+                    const double si = sqk[r][i];
+                    const int pi = q[r] - (ki > 0 ? st[i] : 0);
+                    const double2 t = cur[pi];
+                    acc.b[i] = wacc(acc.b[i], si, t, gk);
+                    nb[r][i] = t;  // slab-m neighbour: the next step's forward needs it
+                    const double2 t2 = cur[pi - (ki > 1 ? st[i] : 0)];
+                    acc.a[Acc<D>::tri(i, i)] = wacc(acc.a[Acc<D>::tri(i, i)], 0.5 * si * sq[ki > 0 ? ki - 1 : 0], t2, gk);
+#pragma unroll
+                    for (int j = i + 1; j < D; ++j) {
+                        const bool ok = ki > 0 && kd[r][j] > 0;
+                        const double2 t3 = cur[ok ? pi - st[j] : q[r]];
+                        acc.a[Acc<D>::tri(i, j)] = wacc(acc.a[Acc<D>::tri(i, j)], si * sqk[r][j], t3, gk);
+                    }
+                }
+                gcur[r] = gnext[r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+            p2v[r] = p1v[r];
+            p1v[r] = vnew[r];
+        }
+    }
+}
+
+// Straight-line sweep of one level with NR points per participating thread
+// (NR compile-time, so the loads and arithmetic of all points of a step form
+// one basic block).  Used for triples with D <= 3 swept whole in LDS.
+//   L = 0: the level-0 slabs in the two-slot ring, with the fused VJP (as
+//          level0_reg);
+//   L > 0: a face level of the sub-lattice {k_0..k_{L-1} = 0}, inside ring
+//          slot 0 (slab m at offset m * stride[L]).
+// `rank`/`tsize` number the participating threads; smode is the ordering
+// between steps: 0 = a single lane (no ordering needed), 1 = one wave,
+// 2 = the team barrier.
+template <int D, bool FUSED, bool WAVE, int L, int NR>
+__device__ __forceinline__ void level_straight(const Job &J, const Team<WAVE> &tm, int rank, int tsize, int smode, double2 *ring,
+                                               const uint32_t *kv, const double *sq, const double *rsq, const double2 *bv,
+                                               const double2 *A, double2 *G, const double2 *g, Acc<D> &acc) {
+    constexpr bool TOP = (L == 0);
+    constexpr bool VJP = TOP && FUSED;
+    const int Sd = int(J.stride[L]);
+    const int sd = J.shape[L];
+    const int S = J.S;
+    int st[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) st[j] = int(J.stride[j]);
+    int q[NR];
+    bool act[NR];
+    int kd[NR][D];
+    double sqk[NR][D];
+    double2 cj[NR][D];
+    double2 nb[NR][D];
+    double2 p1v[NR], p2v[NR], gcur[NR], gnext[NR];
+    const double2 bL = bv[L];
+    const double2 aLL = A[L * D + L];
+    const double2 zero = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int qq = rank + r * tsize;
+        act[r] = qq < Sd;
+        q[r] = act[r] ? qq : Sd - 1;
+        const uint32_t kp = kv[q[r]];
+#pragma unroll
+        for (int j = L + 1; j < D; ++j) {
+            kd[r][j] = digit(kp, J, j);
+            sqk[r][j] = sq[kd[r][j]];
+            cj[r][j] = cscale_promoted(A[L * D + j], sqk[r][j]);
+            nb[r][j] = zero;
+        }
+        p1v[r] = ring[q[r]];
+        p2v[r] = zero;
+        gcur[r] = zero;
+        gnext[r] = zero;
+        if (VJP && sd > 1) {
+            const double2 t = g[int64_t(S) + q[r]];
+            gcur[r] = act[r] ? t : zero;
+        }
+    }
+    for (int m = 1; m < sd; ++m) {
+        double2 *cur = TOP ? ring + (m & 1) * S : ring + m * Sd;
+        const double2 *p1 = TOP ? ring + ((m - 1) & 1) * S : ring + (m - 1) * Sd;
+        const double sm = sq[m], rm = rsq[m];
+        const double sm1 = sq[m - 1];
+        const double2 cLL = cscale_promoted(aLL, sm1);
+        const int64_t gbase = int64_t(m) * Sd;
+        if (VJP && m + 1 < sd) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double2 t = g[gbase + Sd + q[r]];
+                gnext[r] = act[r] ? t : zero;
+            }
+        }
+        double2 v[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            v[r] = cmul(bL, p1v[r]);
+            v[r] = cadd(v[r], cmul(cLL, p2v[r]));
+#pragma unroll
+            for (int j = L + 1; j < D; ++j) {
+                if (smode == 1) {
+                    // one wave holds the whole slab: the neighbour is another lane's register
+                    const int src = q[r] - (kd[r][j] > 0 ? st[j] : 0);
+                    nb[r][j] = make_double2(__shfl(p1v[r].x, src, MM_WAVE), __shfl(p1v[r].y, src, MM_WAVE));
+                } else if (!VJP || m == 1) {
+                    nb[r][j] = p1[q[r] - (kd[r][j] > 0 ? st[j] : 0)];
+                }
+                v[r] = cadd(v[r], cmul(cj[r][j], nb[r][j]));
+            }
+        }
+        cdiv_many<NR>(v, sm, rm);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (act[r]) {
+                cur[q[r]] = v[r];
+                G[gbase + q[r]] = v[r];
+            }
+        }
+        // VJP terms that need no other lane's slab-m value go before the
+        // barrier: the cotangent is pre-scaled by sqrt(k_i) per pivot axis i.
+        double2 u0[NR];
+        if (VJP) {
+            const double h00 = 0.5 * sm1;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double2 gk = gcur[r];
+                u0[r] = make_double2(sm * gk.x, sm * gk.y);
+                acc.c = cfma(v[r], gk, acc.c);
+                acc.b[0] = cfma(p1v[r], u0[r], acc.b[0]);
+                acc.a[0] = cfma(make_double2(h00 * p2v[r].x, h00 * p2v[r].y), u0[r], acc.a[0]);
+#pragma unroll
+                for (int j = 1; j < D; ++j) {
+                    const double w = sqk[r][j];
+                    acc.a[Acc<D>::tri(0, j)] = cfma(make_double2(w * nb[r][j].x, w * nb[r][j].y), u0[r], acc.a[Acc<D>::tri(0, j)]);
+                }
+            }
+        }
+        if (smode == 2)
+            tm.sync();
+        if (VJP) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double2 gk = gcur[r];
+#pragma unroll
+                for (int i = 1; i < D; ++i) {
+                    const int ki = kd[r][i];
+                    const double si = sqk[r][i];
+                    const double2 ui = make_double2(si * gk.x, si * gk.y);
+                    const int pi = q[r] - (ki > 0 ? st[i] : 0);
+                    const double2 t = cur[pi];
+                    acc.b[i] = cfma(t, ui, acc.b[i]);
+                    nb[r][i] = t;  // slab-m neighbour: the next step's forward needs it
+                    const double2 t2 = cur[pi - (ki > 1 ? st[i] : 0)];
+                    const double hi = 0.5 * sq[ki > 0 ? ki - 1 : 0];
+                    acc.a[Acc<D>::tri(i, i)] = cfma(make_double2(hi * t2.x, hi * t2.y), ui, acc.a[Acc<D>::tri(i, i)]);
+#pragma unroll
+                    for (int j = i + 1; j < D; ++j) {
+                        const bool ok = ki > 0 && kd[r][j] > 0;
+                        const double2 t3 = cur[ok ? pi - st[j] : q[r]];
+                        const double w = sqk[r][j];
+                        acc.a[Acc<D>::tri(i, j)] = cfma(make_double2(w * t3.x, w * t3.y), ui, acc.a[Acc<D>::tri(i, j)]);
+                    }
+                }
+                gcur[r] = gnext[r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            p2v[r] = p1v[r];
+            p1v[r] = v[r];
+        }
+    }
+}
+
+// Pick the compile-time point count for level L and run it.
+template <int D, bool FUSED, bool WAVE, int L, int RM>
+__device__ __forceinline__ void level_dispatch(int nr, const Job &J, const Team<WAVE> &tm, int rank, int tsize, int smode,
+                                               double2 *ring, const uint32_t *kv, const double *sq, const double *rsq,
+                                               const double2 *bv, const double2 *A, double2 *G, const double2 *g, Acc<D> &acc) {
+    switch (nr) {
+        case 1:
+            level_straight<D, FUSED, WAVE, L, 1>(J, tm, rank, tsize, smode, ring, kv, sq, rsq, bv, A, G, g, acc);
+            break;
+        case 2:
+            if constexpr (RM >= 2) level_straight<D, FUSED, WAVE, L, 2>(J, tm, rank, tsize, smode, ring, kv, sq, rsq, bv, A, G, g, acc);
+            break;
+        case 3:
+            if constexpr (RM >= 3) level_straight<D, FUSED, WAVE, L, 3>(J, tm, rank, tsize, smode, ring, kv, sq, rsq, bv, A, G, g, acc);
+            break;
+        case 4:
+            if constexpr (RM >= 4) level_straight<D, FUSED, WAVE, L, 4>(J, tm, rank, tsize, smode, ring, kv, sq, rsq, bv, A, G, g, acc);
+            break;
+        case 5:
+            if constexpr (RM >= 5) level_straight<D, FUSED, WAVE, L, 5>(J, tm, rank, tsize, smode, ring, kv, sq, rsq, bv, A, G, g, acc);
+            break;
+        default:
+            break;
+    }
+}
+
+// Face level L (> 0) of a triple with D <= 3 swept whole in LDS: a single
+// lane when the slab is one point, wave 0 alone when it fits one wave,
+// otherwise the whole team.  Ends with every thread of the team past a
+// barrier.
+template <int D, bool FUSED, bool WAVE, int L, int RM>
+__device__ __forceinline__ void face_level(const Job &J, const Team<WAVE> &tm, double2 *ring, const uint32_t *kv, const double *sq,
+                                           const double *rsq, const double2 *bv, const double2 *A, double2 *G, Acc<D> &acc) {
+    const int Sd = int(J.stride[L]);
+    if (J.shape[L] <= 1) return;
+    if (Sd == 1) {
+        if (tm.rank == 0) level_dispatch<D, false, WAVE, L, RM>(1, J, tm, 0, 1, 0, ring, kv, sq, rsq, bv, A, G, nullptr, acc);
+        tm.sync();
+    } else if (WAVE || Sd > MM_WAVE) {
+        const int nr = (Sd + tm.size - 1) / tm.size;
+        level_dispatch<D, false, WAVE, L, RM>(nr, J, tm, tm.rank, tm.size, 2, ring, kv, sq, rsq, bv, A, G, nullptr, acc);
+    } else {
+        if (tm.wave == 0) level_dispatch<D, false, WAVE, L, RM>(1, J, tm, tm.rank, MM_WAVE, 1, ring, kv, sq, rsq, bv, A, G, nullptr, acc);
+        tm.sync();
+    }
+}
+
 // Per-triple shift applied to a template job (uniform batches): offsets of
 // triple k are the template's plus k times these strides.
 struct JobShift {
     int64_t k, ua, ub, uc, ug;
 };
 
+// Copy a job descriptor (from the kernel arguments or the queue table) into
+// the head of the team's LDS region and apply a uniform-batch shift.  Ends
+// with the team's barrier, so the sweep can read it.
+template <bool WAVE>
+__device__ __forceinline__ void load_job(char *lds, const Job *src, const JobShift sh) {
+    const Team<WAVE> tm;
+    Job *dst = reinterpret_cast<Job *>(lds);
+    const uint32_t *s32 = reinterpret_cast<const uint32_t *>(src);
+    uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
+    for (int t = tm.rank; t < int(sizeof(Job) / 4); t += tm.size) d32[t] = s32[t];
+    tm.sync();
+    if (tm.rank == 0 && sh.k > 0) {
+        dst->a_off += sh.k * sh.ua;
+        dst->b_off += sh.k * sh.ub;
+        dst->c_off += sh.k * sh.uc;
+        dst->g_off += sh.k * sh.ug;
+    }
+    tm.sync();
+}
+
 // The sweep.  D is the number of indices; the team is a workgroup (WAVE =
 // false) or a single wave (WAVE = true) that owns `lds` exclusively.  The job
-// descriptor is first copied into the head of the team's LDS region, so that
+// descriptor sits in the head of the team's LDS region (load_job), so that
 // fields indexed by a run-time level are LDS reads rather than private memory.
-template <int D, bool FUSED, bool WAVE>
-__device__ void sweep_lds(const Job *src, const JobShift sh, const Bases &P, const Tables &T, char *lds) {
+template <int D, bool FUSED, bool WAVE, int RM>
+__device__ __forceinline__ void sweep_lds(const Bases P, const Tables T, char *lds) {
     const Team<WAVE> tm;
-    {
-        Job *dst = reinterpret_cast<Job *>(lds);
-        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(src);
-        uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
-        for (int t = tm.rank; t < int(sizeof(Job) / 4); t += tm.size) d32[t] = s32[t];
-        tm.sync();
-        if (tm.rank == 0 && sh.k > 0) {
-            dst->a_off += sh.k * sh.ua;
-            dst->b_off += sh.k * sh.ub;
-            dst->c_off += sh.k * sh.uc;
-            dst->g_off += sh.k * sh.ug;
-        }
-        tm.sync();
-    }
     const Job &J = *reinterpret_cast<const Job *>(lds);
+    MM_PHASE(0);
     const int top = J.top;
     const int S = J.S;
     const int smax = J.smax;
-    const LdsLayout L = lds_layout(D, top, S, smax, FUSED, WAVE ? 1 : tm.nwaves);
+    const int nw = WAVE ? 1 : tm.nwaves;
+    const bool regp = top == 0 && reg_path(S, nw, RM);
+    const LdsLayout L = lds_layout(D, top, S, smax, FUSED, nw, RM);
     double2 *ring = reinterpret_cast<double2 *>(lds + L.ring);
     uint32_t *kv = reinterpret_cast<uint32_t *>(lds + L.kv);
     double2 *coef = reinterpret_cast<double2 *>(lds + L.coef);
     double *sq = reinterpret_cast<double *>(lds + L.sq);
     double *rsq = reinterpret_cast<double *>(lds + L.rsq);
     double2 *bv = reinterpret_cast<double2 *>(lds + L.bvec);
+    double2 *am = reinterpret_cast<double2 *>(lds + L.amat);
     double2 *red = reinterpret_cast<double2 *>(lds + L.red);
 
-    const double2 *A = P.A + J.a_off;
+    const double2 *Ag = P.A + J.a_off;
+    const double2 *A = am;  // the sweep reads A from LDS
     double2 *G = P.G + J.g_off;
     const double2 *g = FUSED ? P.g + J.g_off : nullptr;
 
@@ -226,80 +586,125 @@ __device__ void sweep_lds(const Job *src, const JobShift sh, const Bases &P, con
     }
     if (tm.rank < D) bv[tm.rank] = P.b[J.b_off + tm.rank];
     if (tm.rank == 0) bv[D] = P.c[J.c_off];
+    for (int t = tm.rank; t < D * D; t += tm.size) am[t] = Ag[t];
     const int nrows = coef_rows(D, top);
     for (int t = tm.rank; t < nrows * smax; t += tm.size) {
         int r = t / smax, k = t - r * smax;
-        // row r -> (i, j)
         int i = top, rr = r;
         while (rr >= D - i) {
             rr -= D - i;
             ++i;
         }
         int j = i + rr;
-        double2 a = A[i * D + j];
+        double2 a = Ag[i * D + j];
         coef[t] = (k < J.shape[j]) ? cscale_promoted(a, T.sq[k]) : make_double2(0.0, 0.0);
     }
     for (int f = tm.rank; f < S; f += tm.size) {
         uint32_t kp = 0;
-        int64_t rem = f;
+        int rem = f;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             if (j <= top) continue;
-            int64_t dj = rem / J.stride[j];
-            rem -= dj * J.stride[j];
+            const int sj = int(J.stride[j]);
+            const int dj = rem / sj;
+            rem -= dj * sj;
             kp |= uint32_t(dj) << J.kbit[j];
         }
         kv[f] = kp;
+    }
+    if (tm.rank == 0) {
+        ring[0] = P.c[J.c_off];
+        G[0] = P.c[J.c_off];
     }
     tm.sync();
 
     const double2 c = bv[D];
     Acc<D> acc;
-    if (FUSED) acc.zero();
+    acc.zero();
+    MM_PHASE(1);
 
-    // ---- levels d = D-1 .. top+1 inside ring slot 0
-    if (tm.rank == 0) {
-        ring[0] = c;
-        G[0] = c;
-    }
-    tm.sync();
-    for (int d = D - 1; d > top; --d) {
-        const int Sd = int(J.stride[d]);
-        const int sd = J.shape[d];
-        const double2 bd = bv[d];
-        const double2 *cdd = coef + coef_row(D, top, d, d) * smax;
-        for (int m = 1; m < sd; ++m) {
-            const double sm = sq[m], rm = rsq[m];
-            for (int q = tm.rank; q < Sd; q += tm.size) {
-                const int f = m * Sd + q;
-                const uint32_t kp = kv[f];
-                double2 v = cmul(bd, ring[f - Sd]);
-                if (m >= 2) v = cadd(v, cmul(cdd[m - 1], ring[f - 2 * Sd]));
-#pragma unroll
-                for (int j = 1; j < D; ++j) {
-                    if (j <= d) continue;
-                    const int kj = digit(kp, J, j);
-                    if (kj > 0) v = cadd(v, cmul(coef[coef_row(D, top, d, j) * smax + kj], ring[f - Sd - int(J.stride[j])]));
+    // straight-line path: D <= 3, whole triple in LDS, level 0 in registers
+    constexpr bool STRAIGHT_OK = D <= 3;
+    const bool straight = STRAIGHT_OK && regp;
+    if (straight) {
+        if constexpr (STRAIGHT_OK) {
+            if constexpr (D >= 3) face_level<D, FUSED, WAVE, 2, RM>(J, tm, ring, kv, sq, rsq, bv, A, G, acc);
+            if constexpr (D >= 2) face_level<D, FUSED, WAVE, 1, RM>(J, tm, ring, kv, sq, rsq, bv, A, G, acc);
+        }
+    } else {
+        // ---- levels d = D-1 .. top+1 inside ring slot 0
+        for (int d = D - 1; d > top; --d) {
+            const int Sd = int(J.stride[d]);
+            const int sd = J.shape[d];
+            if (sd <= 1) continue;
+            const double2 bd = bv[d];
+            const double2 *cdd = coef + coef_row(D, top, d, d) * smax;
+            if (Sd == 1) {
+                // every later axis has size 1: a one-index chain, kept in one lane's registers
+                if (tm.rank == 0) {
+                    double2 m1 = ring[0], m2 = make_double2(0.0, 0.0);
+                    for (int m = 1; m < sd; ++m) {
+                        double2 v = cmul(bd, m1);
+                        if (m >= 2) v = cadd(v, cmul(cdd[m - 1], m2));
+                        v = cdiv_by(v, sq[m], rsq[m]);
+                        ring[m] = v;
+                        G[m] = v;
+                        m2 = m1;
+                        m1 = v;
+                    }
                 }
-                v = cdiv_by(v, sm, rm);
-                ring[f] = v;
-                G[f] = v;
+                tm.sync();
+                continue;
             }
-            tm.sync();
+            // slabs of at most one wave's width are swept by wave 0 alone, with
+            // wave-level ordering between steps; the other waves wait at the end
+            const bool one_wave = !WAVE && Sd <= MM_WAVE;
+            if (one_wave && tm.wave != 0) {
+                tm.sync();
+                continue;
+            }
+            const int stride_q = one_wave ? MM_WAVE : tm.size;
+            for (int m = 1; m < sd; ++m) {
+                const double sm = sq[m], rm = rsq[m];
+                for (int q = tm.rank; q < Sd; q += stride_q) {
+                    const int f = m * Sd + q;
+                    const uint32_t kp = kv[f];
+                    double2 v = cmul(bd, ring[f - Sd]);
+                    if (m >= 2) v = cadd(v, cmul(cdd[m - 1], ring[f - 2 * Sd]));
+    #pragma unroll
+                    for (int j = 1; j < D; ++j) {
+                        if (j <= d) continue;
+                        const int kj = digit(kp, J, j);
+                        if (kj > 0) v = cadd(v, cmul(coef[coef_row(D, top, d, j) * smax + kj], ring[f - Sd - int(J.stride[j])]));
+                    }
+                    v = cdiv_by(v, sm, rm);
+                    ring[f] = v;
+                    G[f] = v;
+                }
+                if (one_wave)
+                    wave_sync();
+                else
+                    tm.sync();
+            }
+            if (one_wave) __syncthreads();
         }
     }
-
+    MM_PHASE(2);
     // ---- VJP of the sub-lattice below `top` (all of it sits in ring slot 0)
     if (FUSED) {
-        for (int f = tm.rank; f < S; f += tm.size) {
-            const uint32_t kp = kv[f];
-            // lowest non-zero axis > top is the "pivot" role; vjp_point with lo = top, klo = 0
-            vjp_point<D>(acc, J, sq, kp, top, 0, ring, ring, ring, f, g[f]);
-        }
+        for (int f = tm.rank; f < S; f += tm.size) vjp_point<D>(acc, J, sq, kv[f], top, 0, ring, ring, ring, f, g[f]);
     }
 
-    // ---- level `top`: slabs m = 1 .. shape[top]-1 in the ring
-    {
+    MM_PHASE(3);
+    // ---- level `top`
+    if (straight) {
+        if constexpr (STRAIGHT_OK) {
+            const int nr = (S + tm.size - 1) / tm.size;
+            level_dispatch<D, FUSED, WAVE, 0, RM>(nr, J, tm, tm.rank, tm.size, 2, ring, kv, sq, rsq, bv, A, G, g, acc);
+        }
+    } else if (regp) {
+        level0_reg<D, FUSED, WAVE, RM>(J, tm, ring, kv, sq, rsq, bv, A, G, g, coef, acc);
+    } else {
         const int NR = FUSED ? 3 : 2;
         const int s0 = J.shape[top];
         const double2 b0 = bv[top];
@@ -326,25 +731,26 @@ __device__ void sweep_lds(const Job *src, const JobShift sh, const Bases &P, con
             }
             tm.sync();
             if (FUSED) {
-                for (int q = tm.rank; q < S; q += tm.size)
-                    vjp_point<D>(acc, J, sq, kv[q], top, m, cur, p1, p2, q, g[gbase + q]);
+                for (int q = tm.rank; q < S; q += tm.size) vjp_point<D>(acc, J, sq, kv[q], top, m, cur, p1, p2, q, g[gbase + q]);
             }
         }
     }
 
+    MM_PHASE(4);
     if (FUSED) reduce_and_store<D, WAVE>(acc, J, P, tm, red, c);
     tm.sync();
+    MM_PHASE(5);
 }
 
 // One-index chain in a single lane (D = 1: core.py:108-122 with D = 1).
+// Fields are passed as scalars so that nothing is copied to private memory.
 template <bool FUSED>
-__device__ void chain_lane(const Job &J, const Bases &P, const Tables &T) {
-    const int s0 = J.shape[0];
-    const double2 a = P.A[J.a_off];
-    const double2 b = P.b[J.b_off];
-    const double2 c = P.c[J.c_off];
-    double2 *G = P.G + J.g_off;
-    const double2 *g = FUSED ? P.g + J.g_off : nullptr;
+__device__ __forceinline__ void chain_lane(int s0, int64_t a_off, int64_t b_off, int64_t c_off, int64_t g_off, const Bases P, const Tables T) {
+    const double2 a = P.A[a_off];
+    const double2 b = P.b[b_off];
+    const double2 c = P.c[c_off];
+    double2 *G = P.G + g_off;
+    const double2 *g = FUSED ? P.g + g_off : nullptr;
     double2 m1 = c, m2 = make_double2(0.0, 0.0);
     G[0] = c;
     double2 ab = make_double2(0.0, 0.0), bb = make_double2(0.0, 0.0), cc = make_double2(0.0, 0.0);
@@ -365,16 +771,9 @@ __device__ void chain_lane(const Job &J, const Bases &P, const Tables &T) {
         m1 = v;
     }
     if (FUSED) {
-        if (J.flags & JOB_WRITE_PARTIAL) {
-            double2 *out = P.partial + J.p_off;
-            out[0] = ab;
-            out[1] = bb;
-            out[2] = cc;
-        } else {
-            P.dA[J.a_off] = ab;
-            P.db[J.b_off] = bb;
-            P.dc[J.c_off] = cdiv_py(cc, c);
-        }
+        P.dA[a_off] = ab;
+        P.db[b_off] = bb;
+        P.dc[c_off] = cdiv_py(cc, c);
     }
 }
 

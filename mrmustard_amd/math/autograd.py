@@ -34,9 +34,9 @@ class HermiteRenormalized(torch.autograd.Function):
         G, c = ctx.saved_tensors
         dA, db, dc = st.vanilla_vjp_hip(G, c, grad.conj())
         return (
-            dA.conj().reshape(ctx.a_shape),
-            db.conj().reshape(ctx.b_shape),
-            dc.conj().reshape(ctx.c_shape),
+            dA.conj().resolve_conj().reshape(ctx.a_shape),
+            db.conj().resolve_conj().reshape(ctx.b_shape),
+            dc.conj().resolve_conj().reshape(ctx.c_shape),
             None,
             None,
         )
@@ -57,9 +57,9 @@ class HermiteRenormalizedBatched(torch.autograd.Function):
         G, c = ctx.saved_tensors
         dA, db, dc = st.vanilla_batch_vjp_hip(G, c, grad.conj())
         return (
-            dA.conj().reshape(ctx.a_shape),
-            db.conj().reshape(ctx.b_shape),
-            dc.conj().reshape(ctx.c_shape),
+            dA.conj().resolve_conj().reshape(ctx.a_shape),
+            db.conj().resolve_conj().reshape(ctx.b_shape),
+            dc.conj().resolve_conj().reshape(ctx.c_shape),
             None,
             None,
         )

@@ -1,0 +1,276 @@
+"""The drop-in surface on the GPU: ``math.hermite_renormalized`` dispatch and
+``out`` semantics (test_vanilla.py:165-238), the reference's closed-form
+tests through the GPU path (test_special.py:23-36, test_fock_utils.py), the
+autograd wiring against finite differences and the reference's VJP
+convention (jax_vjps/hermite.py:47-102), and error paths."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from scipy.special import eval_hermite, factorial
+
+from tests.conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
+
+
+def random_triple(n, batch=(), seed=None):
+    """test_vanilla.py:24-35: symmetric A scaled below spectral radius 1."""
+    rng = np.random.RandomState(seed)
+    A = rng.random((*batch, n, n)) + 1j * rng.random((*batch, n, n))
+    A = A + np.swapaxes(A, -1, -2)
+    A /= np.abs(np.linalg.eigvals(A)).max() + 0.2
+    b = rng.random((*batch, n)) + 1j * rng.random((*batch, n))
+    c = rng.random(batch) + 1j * rng.random(batch)
+    return A, b, c
+
+
+def np_(x):
+    import torch
+
+    return x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
+
+
+@pytest.fixture(scope="module")
+def math():
+    import torch
+
+    from mrmustard_amd import math as m
+
+    torch.cuda.set_device(0)
+    return m
+
+
+@pytest.mark.parametrize("stable", [True, False])
+def test_unbatched(math, oracle_mod, stable):
+    A, b, c = random_triple(2, (), seed=673)
+    G = math.hermite_renormalized(A, b, c, (3, 3), stable=stable)
+    assert tuple(G.shape) == (3, 3)
+    ref = (oracle_mod.stable if stable else oracle_mod.vanilla)((3, 3), A, b, c)
+    assert np.array_equal(np_(G), ref)
+
+
+@pytest.mark.parametrize("stable", [True, False])
+def test_unbatched_out_numpy(math, stable):
+    A, b, c = random_triple(2, (), seed=673)
+    out_arr = np.zeros((3, 3), dtype=np.complex128)
+    G = math.hermite_renormalized(A, b, c, (3, 3), stable=stable, out=out_arr)
+    assert out_arr is G
+    assert np.allclose(G, np_(math.hermite_renormalized(A, b, c, (3, 3), stable=stable)))
+
+
+def test_unbatched_out_device(math):
+    import torch
+
+    A, b, c = random_triple(3, (), seed=5)
+    out = torch.zeros((4, 5, 6), dtype=torch.complex128, device="cuda")
+    G = math.hermite_renormalized(A, b, c, (4, 5, 6), out=out)
+    assert G is out
+    assert np.allclose(np_(out), np_(math.hermite_renormalized(A, b, c, (4, 5, 6))))
+
+
+@pytest.mark.parametrize("stable", [True, False])
+def test_b_batched(math, stable):
+    A, b, c = random_triple(2, (2, 1), seed=673)
+    G = math.hermite_renormalized(A[0, 0], b, c[0, 0], (4, 5), stable=stable)
+    assert tuple(G.shape) == (2, 1, 4, 5)
+    assert np.allclose(np_(G[0, 0]), np_(math.hermite_renormalized(A[0, 0], b[0, 0], c[0, 0], (4, 5))))
+    assert np.allclose(np_(G[1, 0]), np_(math.hermite_renormalized(A[0, 0], b[1, 0], c[0, 0], (4, 5))))
+
+
+@pytest.mark.parametrize("stable", [True, False])
+def test_b_batched_out(math, stable):
+    A, b, c = random_triple(2, (2, 1), seed=673)
+    out_arr = np.zeros((2, 1, 4, 5), dtype=np.complex128)
+    G = math.hermite_renormalized(A[0, 0], b, c[0, 0], (4, 5), stable=stable, out=out_arr)
+    assert tuple(G.shape) == (2, 1, 4, 5)
+    assert np.allclose(out_arr[1, 0], np_(math.hermite_renormalized(A[0, 0], b[1, 0], c[0, 0], (4, 5))))
+
+
+@pytest.mark.parametrize("stable", [True, False])
+def test_fully_batched(math, stable):
+    A, b, c = random_triple(2, (2, 1), seed=673)
+    G = math.hermite_renormalized(A, b, c, (4, 5), stable=stable)
+    assert tuple(G.shape) == (2, 1, 4, 5)
+    assert np.allclose(np_(G[0, 0]), np_(math.hermite_renormalized(A[0, 0], b[0, 0], c[0, 0], (4, 5))))
+    assert np.allclose(np_(G[1, 0]), np_(math.hermite_renormalized(A[1, 0], b[1, 0], c[1, 0], (4, 5))))
+
+
+@pytest.mark.parametrize("stable", [True, False])
+def test_fully_batched_out(math, stable):
+    A, b, c = random_triple(2, (2, 1), seed=673)
+    out_arr = np.zeros((2, 1, 4, 5), dtype=np.complex128)
+    G = math.hermite_renormalized(A, b, c, (4, 5), stable=stable, out=out_arr)
+    assert tuple(G.shape) == (2, 1, 4, 5)
+    assert np.allclose(out_arr[1, 0], np_(math.hermite_renormalized(A[1, 0], b[1, 0], c[1, 0], (4, 5))))
+
+
+def test_hermite_polynomials(math):
+    """test_special.py:23-36 through the GPU path."""
+    x = np.arange(-1, 1, 0.1)
+    A = -np.ones([1, 1], dtype=complex)
+    vals = np.array([np_(math.hermite_renormalized(2 * A, 2 * np.array([x0], dtype=complex), 1, (5,))) for x0 in x]).T
+    expected = np.array([eval_hermite(i, x) / np.sqrt(factorial(i)) for i in range(len(vals))])
+    assert np.allclose(vals, expected)
+
+
+def test_coherent_and_squeezed(math):
+    alpha = 0.8 - 0.6j
+    c = np.exp(-0.5 * abs(alpha) ** 2)
+    G = np_(math.hermite_renormalized(np.zeros((1, 1)), np.array([alpha]), c, (20,)))
+    assert np.allclose(G, c * np.array([alpha**n / np.sqrt(factorial(n)) for n in range(20)]))
+    r, phi = 0.9, 0.4
+    G = np_(math.hermite_renormalized(np.array([[-np.tanh(r) * np.exp(1j * phi)]]), np.zeros(1), 1 / np.sqrt(np.cosh(r)), (20,)))
+    n = np.arange(10)
+    assert np.allclose(G[1::2], 0.0)
+    expected = (-np.exp(1j * phi) * np.tanh(r)) ** n * np.sqrt(factorial(2 * n)) / (2.0**n * factorial(n)) / np.sqrt(np.cosh(r))
+    assert np.allclose(G[0::2], expected)
+
+
+def test_stable_flag_from_settings(math, oracle_mod):
+    from mrmustard_amd import settings
+
+    A, b, c = random_triple(2, (), seed=3)
+    A[0, 1] += 0.05  # non-symmetric: stable and vanilla differ
+    with settings(STABLE_FOCK_CONVERSION=True):
+        G = np_(math.hermite_renormalized(A, b, c, (6, 6)))
+    assert np.array_equal(G, oracle_mod.stable((6, 6), A, b, c))
+    assert not np.allclose(G, oracle_mod.vanilla((6, 6), A, b, c))
+
+
+# ------------------------------------------------------------------ autograd
+
+
+def test_autograd_matches_reference_vjp_convention(math, oracle_mod):
+    """torch grads = conj(VJP_ref(conj(grad_out))) for the holomorphic map."""
+    import torch
+
+    A, b, c = random_triple(3, (), seed=9)
+    shape = (4, 5, 3)
+    At = torch.tensor(A, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    ct = torch.tensor(c, device="cuda", requires_grad=True)
+    G = math.hermite_renormalized(At, bt, ct, shape)
+    rng = np.random.default_rng(1)
+    w = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    # L = Re(sum(w * G)):  dL/dG = w (holomorphic), torch's grad_out = conj(w)
+    L = torch.real(torch.sum(torch.tensor(w, device="cuda") * G))
+    L.backward()
+    Gref = oracle_mod.vanilla(shape, A, b, c)
+    dA, db, dc = oracle_mod.vanilla_vjp(Gref, c, w)
+    assert np.allclose(np_(At.grad), np.conj(dA), rtol=1e-12, atol=1e-14)
+    assert np.allclose(np_(bt.grad), np.conj(db), rtol=1e-12, atol=1e-14)
+    assert np.allclose(np_(ct.grad), np.conj(dc), rtol=1e-12, atol=1e-14)
+
+
+def test_autograd_vs_finite_differences(math, oracle_mod):
+    """Gradient of a real loss w.r.t. the real and imaginary parts of b."""
+    import torch
+
+    A, b, c = random_triple(2, (), seed=10)
+    shape = (7, 6)
+    rng = np.random.default_rng(2)
+    w = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+    def loss_np(bv):
+        return float(np.real(np.sum(w * oracle_mod.vanilla(shape, A, bv, c))))
+
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    L = torch.real(torch.sum(torch.tensor(w, device="cuda") * math.hermite_renormalized(A, bt, c, shape)))
+    L.backward()
+    h = 1e-6
+    for i in range(2):
+        e = np.zeros(2)
+        e[i] = h
+        d_re = (loss_np(b + e) - loss_np(b - e)) / (2 * h)
+        d_im = (loss_np(b + 1j * e) - loss_np(b - 1j * e)) / (2 * h)
+        # torch's complex gradient of a real loss is dL/dRe + i dL/dIm
+        assert np.isclose(np_(bt.grad)[i], d_re + 1j * d_im, rtol=1e-6, atol=1e-8)
+
+
+def test_autograd_batched(math, oracle_mod):
+    import torch
+
+    A, b, c = random_triple(2, (3,), seed=11)
+    shape = (5, 4)
+    At = torch.tensor(A, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    ct = torch.tensor(c, device="cuda", requires_grad=True)
+    G = math.hermite_renormalized(At, bt, ct, shape)
+    rng = np.random.default_rng(3)
+    w = rng.normal(size=(3, *shape)) + 1j * rng.normal(size=(3, *shape))
+    torch.real(torch.sum(torch.tensor(w, device="cuda") * G)).backward()
+    for k in range(3):
+        dA, db, dc = oracle_mod.vanilla_vjp(oracle_mod.vanilla(shape, A[k], b[k], c[k]), c[k], w[k])
+        assert np.allclose(np_(At.grad)[k], np.conj(dA), rtol=1e-12, atol=1e-14)
+        assert np.allclose(np_(bt.grad)[k], np.conj(db), rtol=1e-12, atol=1e-14)
+        assert np.allclose(np_(ct.grad)[k], np.conj(dc), rtol=1e-12, atol=1e-14)
+
+
+def test_out_rejected_with_grad(math):
+    import torch
+
+    A, b, c = random_triple(2, (), seed=12)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    with pytest.raises(ValueError):
+        math.hermite_renormalized(A, bt, c, (3, 3), out=np.zeros((3, 3), complex))
+
+
+# ---------------------------------------------------------------- errors
+
+
+def test_shape_and_d_errors(math):
+    A, b, c = random_triple(2, (), seed=13)
+    with pytest.raises(ValueError):
+        math.hermite_renormalized(A, b, c, (3, 3, 3))  # D mismatch
+    with pytest.raises(ValueError):
+        math.hermite_renormalized(A, b, c, (3, 0))  # empty axis
+    with pytest.raises(ValueError):
+        math.hermite_renormalized(A, b, c, (3, 100000))  # beyond the SQRT table
+    A9, b9, c9 = random_triple(9, (), seed=14)
+    with pytest.raises(ValueError):
+        math.hermite_renormalized(A9, b9, c9, (2,) * 9)  # D > 8
+
+
+def test_workspace_errors_surface_as_exceptions(math):
+    """A C-ABI call with a missing workspace fails with a clear error."""
+    import ctypes
+
+    import torch
+
+    from mrmustard_amd import _lib
+
+    L = _lib.load()
+    shape = np.array([4, 80, 80], dtype=np.int64)
+    G = torch.zeros((4, 80, 80), dtype=torch.complex128, device="cuda")
+    c = torch.ones((), dtype=torch.complex128, device="cuda")
+    rc = L.mm_hermite_vjp(
+        ctypes.c_void_p(G.data_ptr()), ctypes.c_void_p(c.data_ptr()), ctypes.c_void_p(G.data_ptr()), shape.ctypes.data, 3,
+        ctypes.c_void_p(G.data_ptr()), ctypes.c_void_p(G.data_ptr()), ctypes.c_void_p(G.data_ptr()), None, 0, None,
+    )
+    assert rc == _lib.MM_ERR_WORKSPACE
+    assert b"workspace" in L.mm_last_error()
+
+
+def test_shim_backend_numbers(oracle_mod):
+    """The wrapper installed into a MrMustard-style manager returns numpy
+    results equal to the oracle and honours `out`."""
+    from mrmustard_amd import shim
+
+    class NumpyBackend:
+        name = "numpy"
+
+    class Manager:
+        _backend = NumpyBackend()
+
+    m = Manager()
+    shim.install(m)
+    A, b, c = random_triple(3, (), seed=21)
+    G = m._backend.hermite_renormalized(A, b, c, (5, 4, 6))
+    assert isinstance(G, np.ndarray) and np.array_equal(G, oracle_mod.vanilla((5, 4, 6), A, b, c))
+    out = np.zeros((5, 4, 6), complex)
+    assert m._backend.hermite_renormalized(A, b, c, (5, 4, 6), out=out) is out
+    Ab, bb, cb = random_triple(2, (3,), seed=22)
+    Gb = m._backend.hermite_renormalized_batched(Ab, bb, cb, (4, 5))
+    assert np.array_equal(Gb, oracle_mod.vanilla_batch((4, 5), Ab, bb, cb))
