@@ -96,6 +96,17 @@ int mm_hermite_batch_fused(const double *A, const double *b, const double *c, co
                            int64_t B, const int64_t *shape, int D, double *G, double *dA,
                            double *db, double *dc, void *workspace, size_t ws_bytes, void *stream);
 
+/* VJP with recomputation: (dA, db, dc) from (A, b, c, dLdG) without G.  G is
+ * recomputed slab by slab (axis-0 slabs) in a window of three slabs held in
+ * the workspace, so resident memory is 3 * prod(shape[1:]) complex values
+ * instead of prod(shape).  Same results as mm_hermite_vjp applied to
+ * mm_hermite_fwd's G (vanilla_vjp_numba, vanilla/gradients.py:25-82, of
+ * vanilla_numba's output, core.py:25-124). */
+size_t mm_hermite_vjp_recompute_workspace(const int64_t *shape, int D);
+int mm_hermite_vjp_recompute(const double *A, const double *b, const double *c, const double *dLdG,
+                             const int64_t *shape, int D, double *dA, double *db, double *dc,
+                             void *workspace, size_t ws_bytes, void *stream);
+
 /* ------------------------------------------------- work-queue batch (plans)
  *
  * A plan describes n triples of uneven mode count D_k <= 8 and shape_k.

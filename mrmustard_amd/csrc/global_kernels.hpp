@@ -87,6 +87,130 @@ __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
     }
 }
 
+// Level-0 slab m with the three slabs in separate buffers (a rolling window
+// that never holds the whole lattice): cur <- f(p1, p2).
+struct Slab0Args {
+    Geom g;
+    const double2 *A, *b;
+    const double2 *p1, *p2;  // slabs m-1 and m-2 (p2 unused for m = 1)
+    double2 *cur;
+    const double *sq, *rsq;
+    int m;
+};
+
+template <int D>
+__global__ __launch_bounds__(256) void fwd_slab0_kernel(Slab0Args a) {
+    const int m = a.m;
+    const int64_t S = a.g.stride[0];
+    const int64_t q0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t inc = int64_t(gridDim.x) * blockDim.x;
+    if (q0 >= S) return;
+    int k[D], ki[D];
+    digits_of<D>(q0, a.g, 1, k);
+    digits_of<D>(inc < S ? inc : 0, a.g, 1, ki);
+    const double2 b0 = a.b[0];
+    const double2 c00 = cscale_promoted(a.A[0], a.sq[m - 1]);
+    const double sm = a.sq[m], rm = a.rsq[m];
+    double2 A0j[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) A0j[j] = (j > 0) ? a.A[j] : make_double2(0.0, 0.0);
+    for (int64_t q = q0; q < S; q += inc) {
+        double2 v = cmul(b0, a.p1[q]);
+        if (m >= 2) v = cadd(v, cmul(c00, a.p2[q]));
+#pragma unroll
+        for (int j = 1; j < D; ++j)
+            if (k[j] > 0) v = cadd(v, cmul(cscale_promoted(A0j[j], a.sq[k[j]]), a.p1[q - a.g.stride[j]]));
+        a.cur[q] = cdiv_by(v, sm, rm);
+        if (q + inc < S) digits_add<D>(k, ki, a.g, 1);
+    }
+}
+
+// VJP terms of the points of level-0 slab m (k_0 = m), the three slabs given
+// as separate buffers; each block adds its sums into its own partial row
+// (launches are stream-ordered, so the order of additions is fixed).
+struct SlabVjpArgs {
+    Geom g;
+    const double2 *cur, *p1, *p2, *gg;
+    double2 *partial;
+    const double *sq;
+    int m;
+};
+
+template <int D>
+__global__ __launch_bounds__(256) void vjp_slab_kernel(SlabVjpArgs a) {
+    constexpr int NA = D * (D + 1) / 2 + D + 1;
+    constexpr int NT = D * (D + 1) / 2;
+    const int m = a.m;
+    const int64_t S = a.g.stride[0];
+    double2 acc[NA];
+#pragma unroll
+    for (int e = 0; e < NA; ++e) acc[e] = make_double2(0.0, 0.0);
+    const int64_t q0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t inc = int64_t(gridDim.x) * blockDim.x;
+    if (q0 < S) {
+        int k[D], ki[D];
+        digits_of<D>(q0, a.g, 1, k);
+        digits_of<D>(inc < S ? inc : 0, a.g, 1, ki);
+        const double sm = a.sq[m];
+        const double w00 = 0.5 * sm * a.sq[m > 0 ? m - 1 : 0];
+        for (int64_t q = q0; q < S; q += inc) {
+            const double2 gk = a.gg[q];
+            acc[NA - 1] = cfma(a.cur[q], gk, acc[NA - 1]);
+            if (m >= 1) {
+                const double2 u0 = make_double2(sm * gk.x, sm * gk.y);
+                acc[NT] = cfma(a.p1[q], u0, acc[NT]);
+                if (m >= 2) acc[0] = caxpy(w00, cmul(a.p2[q], gk), acc[0]);
+#pragma unroll
+                for (int j = 1; j < D; ++j)
+                    if (k[j] > 0) acc[j] = cfma(make_double2(a.sq[k[j]] * a.p1[q - a.g.stride[j]].x, a.sq[k[j]] * a.p1[q - a.g.stride[j]].y), u0, acc[j]);
+            }
+#pragma unroll
+            for (int i = 1; i < D; ++i) {
+                const int kk = k[i];
+                if (kk == 0) continue;
+                const double si = a.sq[kk];
+                const double2 ui = make_double2(si * gk.x, si * gk.y);
+                const int64_t p = q - a.g.stride[i];
+                acc[NT + i] = cfma(a.cur[p], ui, acc[NT + i]);
+                const int tii = i * D - i * (i - 1) / 2;
+                if (kk > 1) {
+                    const double h = 0.5 * a.sq[kk - 1];
+                    const double2 t2 = a.cur[p - a.g.stride[i]];
+                    acc[tii] = cfma(make_double2(h * t2.x, h * t2.y), ui, acc[tii]);
+                }
+#pragma unroll
+                for (int j = i + 1; j < D; ++j) {
+                    if (k[j] > 0) {
+                        const double w = a.sq[k[j]];
+                        const double2 t3 = a.cur[p - a.g.stride[j]];
+                        acc[tii + (j - i)] = cfma(make_double2(w * t3.x, w * t3.y), ui, acc[tii + (j - i)]);
+                    }
+                }
+            }
+            if (q + inc < S) digits_add<D>(k, ki, a.g, 1);
+        }
+    }
+    __shared__ double2 red[(256 / MM_WAVE) * NA];
+    wave_sum_dpp<NA>(acc);
+    const int lane = threadIdx.x & (MM_WAVE - 1), w = threadIdx.x / MM_WAVE;
+    if (lane == MM_WAVE - 1) {
+#pragma unroll
+        for (int e = 0; e < NA; ++e) red[w * NA + e] = acc[e];
+    }
+    __syncthreads();
+    if (threadIdx.x < NA) {
+        double2 t = red[threadIdx.x];
+        for (int ww = 1; ww < int(blockDim.x / MM_WAVE); ++ww) t = cadd(t, red[ww * NA + threadIdx.x]);
+        double2 *row = a.partial + int64_t(blockIdx.x) * NA;
+        row[threadIdx.x] = cadd(row[threadIdx.x], t);
+    }
+}
+
+__global__ void zero_c128_kernel(double2 *p, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = make_double2(0.0, 0.0);
+}
+
 // ------------------------------------------------------------------- the VJP
 // gradients.py:64-80 as a parallel reduction: each thread accumulates the
 // terms of its points (fused multiply-adds), a block reduces through shuffles

@@ -468,6 +468,142 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
     return MM_OK;
 }
 
+// ------------------------------------------------------- recompute VJP path
+//
+// dL/dA, dL/db, dL/dc from (A, b, c, dL/dG) without the whole of G in
+// memory: the face k_0 = 0 is computed into window slot 0, then each level-0
+// slab m is recomputed into slot m % 3 from slots (m-1) % 3 and (m-2) % 3 and
+// its VJP terms are added to per-block partial sums.  Resident memory is
+// three slabs (3 * stride[0] * 16 B) instead of the lattice.  Lattices whose
+// slabs fit in LDS take the fused LDS sweep with a full G in the workspace.
+
+int recompute_blocks(const Shape &S, const DevState *ds) {
+    const int64_t S0 = S.st[0];
+    return int(std::max<int64_t>(1, std::min<int64_t>((S0 + 1023) / 1024, int64_t(ds->num_cu) * 8)));
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+size_t recompute_ws_bytes(const Shape &S, const DevState *ds) {
+    int T = 64;
+    const int top = (S.D == 1) ? 0 : choose_top(S, true, ds->max_lds, lds_maxt(S.D, true), &T);
+    if (S.D == 1 || top == 0) return align256(size_t(S.N) * 16);  // small: full G, fused sweep
+    return 3 * align256(size_t(S.st[0]) * 16) + size_t(recompute_blocks(S, ds)) * nacc_of(S.D) * 16 + 256;
+}
+
+int vjp_recompute(const Shape &S, const double *A, const double *b, const double *c, const double *g, double *dA, double *db,
+                  double *dc, void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s) {
+    const size_t need = recompute_ws_bytes(S, ds);
+    if (!ws || ws_bytes < need) return fail(MM_ERR_WORKSPACE, "recompute workspace of %zu bytes needed, %zu given", need, ws_bytes);
+    int T = 64;
+    const int topf = (S.D == 1) ? 0 : choose_top(S, true, ds->max_lds, lds_maxt(S.D, true), &T);
+    if (S.D == 1 || topf == 0)
+        return vanilla_uniform(S, 1, A, b, c, g, reinterpret_cast<double *>(ws), dA, db, dc, nullptr, 0, ds, s);
+
+    const size_t slab_bytes = align256(size_t(S.st[0]) * 16);
+    char *base = reinterpret_cast<char *>(ws);
+    double2 *slot[3] = {reinterpret_cast<double2 *>(base), reinterpret_cast<double2 *>(base + slab_bytes),
+                        reinterpret_cast<double2 *>(base + 2 * slab_bytes)};
+    double2 *partial = reinterpret_cast<double2 *>(base + 3 * slab_bytes);
+    const int nblk = recompute_blocks(S, ds);
+    const int NA = nacc_of(S.D);
+    hipLaunchKernelGGL(zero_c128_kernel, dim3(64), dim3(256), 0, s, partial, int64_t(nblk) * NA);
+    LAUNCH_CHECK("zero_c128_kernel");
+
+    // 1. the face {k_0 = 0} into slot 0: LDS sweep from `top` >= 1 (it writes
+    //    only flat indices below stride[top-1] <= stride[0]), then slab
+    //    launches for levels top-1 .. 1
+    int top = choose_top(S, false, ds->max_lds, lds_maxt(S.D, false), &T);
+    if (top == 0) {
+        top = 1;
+        T = team_threads(S.st[1], lds_maxt(S.D, false));
+        if (lds_bytes(S, 1, false, T / MM_WAVE, kernel_dmax(S.D)) > ds->max_lds || !packed_bits_ok(S, 1))
+            return fail(MM_ERR_UNSUPPORTED, "recompute: face does not fit in LDS");
+    }
+    LdsArgs a;
+    memset(&a, 0, sizeof(a));
+    a.P = bases(A, b, c, nullptr, reinterpret_cast<double *>(slot[0]), nullptr, nullptr, nullptr, nullptr);
+    a.T.sq = ds->sq;
+    a.T.rsq = ds->rsq;
+    a.mode = MODE_UNIFORM;
+    a.n = 1;
+    a.tmpl = make_job(S, top, KIND_BLOCK);
+    int rc = launch_lds(a, S.D, false, 1, T, lds_bytes(S, top, false, T / MM_WAVE, kernel_dmax(S.D)), s);
+    if (rc) return rc;
+    SlabArgs sa;
+    sa.g = geom_of(S);
+    sa.A = reinterpret_cast<const double2 *>(A);
+    sa.b = reinterpret_cast<const double2 *>(b);
+    sa.G = slot[0];
+    sa.sq = ds->sq;
+    sa.rsq = ds->rsq;
+    for (int d = top - 1; d >= 1; --d) {
+        for (int m = 1; m < S.s[d]; ++m) {
+            sa.d = d;
+            sa.m = m;
+            rc = launch_slab(sa, S.D, S.st[d], ds->num_cu, s);
+            if (rc) return rc;
+        }
+    }
+    // 2. level 0: VJP of slab 0, then recompute slab m and add its VJP terms
+    const int64_t S0 = S.st[0];
+    const int64_t fblocks = std::max<int64_t>(1, std::min<int64_t>((S0 + 255) / 256, int64_t(ds->num_cu) * 16));
+    for (int m = 0; m < S.s[0]; ++m) {
+        if (m >= 1) {
+            Slab0Args f;
+            f.g = geom_of(S);
+            f.A = reinterpret_cast<const double2 *>(A);
+            f.b = reinterpret_cast<const double2 *>(b);
+            f.p1 = slot[(m - 1) % 3];
+            f.p2 = m >= 2 ? slot[(m - 2) % 3] : slot[(m - 1) % 3];
+            f.cur = slot[m % 3];
+            f.sq = ds->sq;
+            f.rsq = ds->rsq;
+            f.m = m;
+            switch (S.D) {
+#define CASE(DD) \
+    case DD: hipLaunchKernelGGL((fwd_slab0_kernel<DD>), dim3(unsigned(fblocks)), dim3(256), 0, s, f); break;
+                CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+                default:
+                    return fail(MM_ERR_INVALID, "bad D");
+            }
+            LAUNCH_CHECK("fwd_slab0_kernel");
+        }
+        SlabVjpArgs v;
+        v.g = geom_of(S);
+        v.cur = slot[m % 3];
+        v.p1 = m >= 1 ? slot[(m - 1) % 3] : slot[0];
+        v.p2 = m >= 2 ? slot[(m - 2) % 3] : slot[0];
+        v.gg = reinterpret_cast<const double2 *>(g) + int64_t(m) * S0;
+        v.partial = partial;
+        v.sq = ds->sq;
+        v.m = m;
+        switch (S.D) {
+#define CASE(DD) \
+    case DD: hipLaunchKernelGGL((vjp_slab_kernel<DD>), dim3(unsigned(nblk)), dim3(256), 0, s, v); break;
+            CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+            default:
+                return fail(MM_ERR_INVALID, "bad D");
+        }
+        LAUNCH_CHECK("vjp_slab_kernel");
+    }
+    // 3. sum the block partials and finish (symmetrise, dc = sum / c)
+    const double2 *c2 = reinterpret_cast<const double2 *>(c);
+    double2 *dA2 = reinterpret_cast<double2 *>(dA), *db2 = reinterpret_cast<double2 *>(db), *dc2 = reinterpret_cast<double2 *>(dc);
+    switch (S.D) {
+#define CASE(DD) \
+    case DD: hipLaunchKernelGGL((vjp_finalize<DD>), dim3(1), dim3(64), 0, s, partial, nblk, c2, dA2, db2, dc2); break;
+        CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad D");
+    }
+    LAUNCH_CHECK("vjp_finalize");
+    return MM_OK;
+}
+
 // ------------------------------------------------------------------- plans
 
 struct GlobalJob {
@@ -859,6 +995,26 @@ int mm_plan_run_graph(mm_plan *p, const double *A, const double *b, const double
     }
     HIP_TRY(hipGraphLaunch(p->exec, reinterpret_cast<hipStream_t>(stream)));
     return MM_OK;
+}
+
+size_t mm_hermite_vjp_recompute_workspace(const int64_t *shape, int D) {
+    Shape S;
+    if (make_shape(shape, D, &S)) return 0;
+    DevState *ds = nullptr;
+    if (device_state(&ds)) return 0;
+    return recompute_ws_bytes(S, ds);
+}
+
+int mm_hermite_vjp_recompute(const double *A, const double *b, const double *c, const double *dLdG, const int64_t *shape, int D,
+                             double *dA, double *db, double *dc, void *workspace, size_t ws_bytes, void *stream) {
+    Shape S;
+    int rc = make_shape(shape, D, &S);
+    if (rc) return rc;
+    if (!A || !b || !c || !dLdG || !dA || !db || !dc) return fail(MM_ERR_INVALID, "NULL array pointer");
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    return vjp_recompute(S, A, b, c, dLdG, dA, db, dc, workspace, ws_bytes, ds, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -12,6 +12,8 @@ ABI of libmmhip.so on the current stream:
 - ``vanilla_fused_hip`` / ``vanilla_batch_fused_hip``: forward and VJP in one
   sweep when the cotangent is known up front (no reference counterpart; it
   equals the two calls in sequence).
+- ``vanilla_vjp_recompute_hip``: the VJP from (A, b, c) without G, recomputing
+  the lattice slab by slab in a bounded window.
 
 The VJPs use the reference's holomorphic convention: ``dLdG`` is not
 conjugated, and dA is returned symmetrised, (M + M^T)/2.
@@ -29,6 +31,7 @@ from .. import _lib
 
 __all__ = [
     "stable_hip",
+    "vanilla_vjp_recompute_hip",
     "vanilla_batch_fused_hip",
     "vanilla_batch_hip",
     "vanilla_batch_vjp_hip",
@@ -231,3 +234,26 @@ def vanilla_batch_fused_hip(shape, A, b, c, dLdG, out=None):
     )
     _lib.check(rc, "mm_hermite_batch_fused")
     return G, dA, db, dc
+
+
+def vanilla_vjp_recompute_hip(shape, A, b, c, dLdG):
+    """(dL/dA, dL/db, dL/dc) of the forward without G: the lattice is recomputed
+    slab by slab in a three-slab window (mm_hermite_vjp_recompute).  Equal to
+    ``vanilla_vjp_hip(vanilla_hip(shape, A, b, c), c, dLdG)``."""
+    L = _lib.load()
+    A, b, c, g = as_device_c128(A), as_device_c128(b), as_device_c128(c), as_device_c128(dLdG)
+    D = int(b.shape[-1])
+    shape = _check_shape(shape, D)
+    if tuple(g.shape) != shape:
+        raise ValueError("dLdG must have the requested shape")
+    s = _shape_arr(shape)
+    dA = torch.empty((D, D), dtype=C128, device=g.device)
+    db = torch.empty((D,), dtype=C128, device=g.device)
+    dc = torch.empty((), dtype=C128, device=g.device)
+    ws = _workspace(L.mm_hermite_vjp_recompute_workspace(s.ctypes.data, D))
+    rc = L.mm_hermite_vjp_recompute(
+        _ptr(A), _ptr(b), _ptr(c), _ptr(g), s.ctypes.data, D, _ptr(dA), _ptr(db), _ptr(dc), _ptr(ws),
+        0 if ws is None else ws.numel(), _stream(),
+    )
+    _lib.check(rc, "mm_hermite_vjp_recompute")
+    return dA, db, dc

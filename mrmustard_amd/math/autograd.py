@@ -63,3 +63,29 @@ class HermiteRenormalizedBatched(torch.autograd.Function):
             None,
             None,
         )
+
+
+class HermiteRenormalizedRecompute(torch.autograd.Function):
+    """As HermiteRenormalized, but autograd keeps only (A, b, c): the backward
+    recomputes the lattice slab by slab in a three-slab window
+    (``vanilla_vjp_recompute_hip``), so G need not stay resident between the
+    forward and the backward."""
+
+    @staticmethod
+    def forward(ctx, A, b, c, shape, stable):
+        G = st.stable_hip(shape, A, b, c) if stable else st.vanilla_hip(shape, A, b, c)
+        ctx.save_for_backward(A, b, c)
+        ctx.shape = tuple(shape)
+        return G
+
+    @staticmethod
+    def backward(ctx, grad):
+        A, b, c = ctx.saved_tensors
+        dA, db, dc = st.vanilla_vjp_recompute_hip(ctx.shape, A, b, c, grad.conj())
+        return (
+            dA.conj().resolve_conj().reshape(A.shape),
+            db.conj().resolve_conj().reshape(b.shape),
+            dc.conj().resolve_conj().reshape(c.shape),
+            None,
+            None,
+        )

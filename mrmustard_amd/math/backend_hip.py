@@ -22,7 +22,8 @@ import numpy as np
 import torch
 
 from ..lattice import strategies as st
-from .autograd import HermiteRenormalized, HermiteRenormalizedBatched
+from ..settings import settings
+from .autograd import HermiteRenormalized, HermiteRenormalizedBatched, HermiteRenormalizedRecompute
 
 
 def _needs_grad(*xs) -> bool:
@@ -99,7 +100,8 @@ class BackendHip:
             if out is not None:
                 raise ValueError("out is not supported when gradients are tracked")
             A, b, c = st.as_device_c128(A), st.as_device_c128(b), st.as_device_c128(c)
-            return HermiteRenormalized.apply(A, b, c, tuple(int(s) for s in shape), bool(stable))
+            fn = HermiteRenormalizedRecompute if settings.HERMITE_RECOMPUTE_VJP else HermiteRenormalized
+            return fn.apply(A, b, c, tuple(int(s) for s in shape), bool(stable))
         dev_out = out if isinstance(out, torch.Tensor) else None
         G = (st.stable_hip if stable else st.vanilla_hip)(shape, A, b, c, out=dev_out)
         return self._deliver(G, out)

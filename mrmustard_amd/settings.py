@@ -22,6 +22,9 @@ class Settings:
             inst._stack = []
             # settings.py:101 -- ORed into `stable` by backend_manager.py:682
             inst.STABLE_FOCK_CONVERSION = False
+            # not in the reference: autograd keeps only (A, b, c) and the backward
+            # recomputes the lattice slab by slab (bounded memory)
+            inst.HERMITE_RECOMPUTE_VJP = False
             # settings.py:79, :67-74 (consumed by callers that pick a shape)
             inst.DEFAULT_FOCK_SIZE = 50
             inst.AUTOSHAPE_PROBABILITY = 0.99999

@@ -205,3 +205,34 @@ def test_plan_tunings(oracle_mod, st, tune):
 def test_plan_graph_replay(oracle_mod, st):
     print("plan graph replay", flush=True)
     _uneven(oracle_mod, 150, 23, True, graph=True)
+
+
+@pytest.mark.parametrize("shape", [(7,), (9, 11), (30, 30, 30), (5, 80, 80), (3, 5, 40, 45), (3, 4, 4, 4, 4, 4, 4, 4)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_vjp_recompute(st, oracle_mod, shape):
+    """The windowed recompute VJP equals the VJP of the stored forward."""
+    rng = np.random.default_rng(sum(shape) + 5000)
+    A, b, c = triple(rng, len(shape))
+    g = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    print(f"vjp recompute shape={shape}", flush=True)
+    got = st.vanilla_vjp_recompute_hip(shape, A, b, c, g)
+    ref = oracle_mod.vanilla(shape, A, b, c)
+    for x, r in zip(got, oracle_mod.vanilla_vjp(ref, c, g, kahan=True)):
+        assert rel(np_(x), r) <= TOL
+
+
+def test_autograd_recompute_setting(oracle_mod, st):
+    import torch
+
+    from mrmustard_amd import math, settings
+
+    rng = np.random.default_rng(77)
+    A, b, c = triple(rng, 3)
+    shape = (5, 80, 80)
+    w = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    with settings(HERMITE_RECOMPUTE_VJP=True):
+        G = math.hermite_renormalized(A, bt, c, shape)
+    torch.real(torch.sum(torch.tensor(w, device="cuda") * G)).backward()
+    _, db, _ = oracle_mod.vanilla_vjp(oracle_mod.vanilla(shape, A, b, c), c, w)
+    assert rel(np_(bt.grad), np.conj(db)) <= TOL
