@@ -125,87 +125,6 @@ __global__ __launch_bounds__(256) void fwd_slab0_kernel(Slab0Args a) {
     }
 }
 
-// VJP terms of the points of level-0 slab m (k_0 = m), the three slabs given
-// as separate buffers; each block adds its sums into its own partial row
-// (launches are stream-ordered, so the order of additions is fixed).
-struct SlabVjpArgs {
-    Geom g;
-    const double2 *cur, *p1, *p2, *gg;
-    double2 *partial;
-    const double *sq;
-    int m;
-};
-
-template <int D>
-__global__ __launch_bounds__(256) void vjp_slab_kernel(SlabVjpArgs a) {
-    constexpr int NA = D * (D + 1) / 2 + D + 1;
-    constexpr int NT = D * (D + 1) / 2;
-    const int m = a.m;
-    const int64_t S = a.g.stride[0];
-    double2 acc[NA];
-#pragma unroll
-    for (int e = 0; e < NA; ++e) acc[e] = make_double2(0.0, 0.0);
-    const int64_t q0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t inc = int64_t(gridDim.x) * blockDim.x;
-    if (q0 < S) {
-        int k[D], ki[D];
-        digits_of<D>(q0, a.g, 1, k);
-        digits_of<D>(inc < S ? inc : 0, a.g, 1, ki);
-        const double sm = a.sq[m];
-        const double w00 = 0.5 * sm * a.sq[m > 0 ? m - 1 : 0];
-        for (int64_t q = q0; q < S; q += inc) {
-            const double2 gk = a.gg[q];
-            acc[NA - 1] = cfma(a.cur[q], gk, acc[NA - 1]);
-            if (m >= 1) {
-                const double2 u0 = make_double2(sm * gk.x, sm * gk.y);
-                acc[NT] = cfma(a.p1[q], u0, acc[NT]);
-                if (m >= 2) acc[0] = caxpy(w00, cmul(a.p2[q], gk), acc[0]);
-#pragma unroll
-                for (int j = 1; j < D; ++j)
-                    if (k[j] > 0) acc[j] = cfma(make_double2(a.sq[k[j]] * a.p1[q - a.g.stride[j]].x, a.sq[k[j]] * a.p1[q - a.g.stride[j]].y), u0, acc[j]);
-            }
-#pragma unroll
-            for (int i = 1; i < D; ++i) {
-                const int kk = k[i];
-                if (kk == 0) continue;
-                const double si = a.sq[kk];
-                const double2 ui = make_double2(si * gk.x, si * gk.y);
-                const int64_t p = q - a.g.stride[i];
-                acc[NT + i] = cfma(a.cur[p], ui, acc[NT + i]);
-                const int tii = i * D - i * (i - 1) / 2;
-                if (kk > 1) {
-                    const double h = 0.5 * a.sq[kk - 1];
-                    const double2 t2 = a.cur[p - a.g.stride[i]];
-                    acc[tii] = cfma(make_double2(h * t2.x, h * t2.y), ui, acc[tii]);
-                }
-#pragma unroll
-                for (int j = i + 1; j < D; ++j) {
-                    if (k[j] > 0) {
-                        const double w = a.sq[k[j]];
-                        const double2 t3 = a.cur[p - a.g.stride[j]];
-                        acc[tii + (j - i)] = cfma(make_double2(w * t3.x, w * t3.y), ui, acc[tii + (j - i)]);
-                    }
-                }
-            }
-            if (q + inc < S) digits_add<D>(k, ki, a.g, 1);
-        }
-    }
-    __shared__ double2 red[(256 / MM_WAVE) * NA];
-    wave_sum_dpp<NA>(acc);
-    const int lane = threadIdx.x & (MM_WAVE - 1), w = threadIdx.x / MM_WAVE;
-    if (lane == MM_WAVE - 1) {
-#pragma unroll
-        for (int e = 0; e < NA; ++e) red[w * NA + e] = acc[e];
-    }
-    __syncthreads();
-    if (threadIdx.x < NA) {
-        double2 t = red[threadIdx.x];
-        for (int ww = 1; ww < int(blockDim.x / MM_WAVE); ++ww) t = cadd(t, red[ww * NA + threadIdx.x]);
-        double2 *row = a.partial + int64_t(blockIdx.x) * NA;
-        row[threadIdx.x] = cadd(row[threadIdx.x], t);
-    }
-}
-
 __global__ void zero_c128_kernel(double2 *p, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
         p[i] = make_double2(0.0, 0.0);
@@ -224,56 +143,186 @@ struct VjpArgs {
     int nchunks;
 };
 
-template <int D>
+// Accumulators of one group of pivot rows [R0, R1): dA(i, j) for R0 <= i < R1
+// and all j >= i, db_i for those rows, and dc with the first group.  Large D
+// splits the rows over several launches so each thread holds few sums.
+template <int D, int R0, int R1>
+struct RowGroup {
+    static constexpr int NT = D * (D + 1) / 2;
+    static constexpr int NA = NT + D + 1;
+    static constexpr int NROWA = (R1 - R0) * D - (R1 * (R1 - 1) - R0 * (R0 - 1)) / 2;  // sum_{i=R0}^{R1-1} (D - i)
+    static constexpr int NL = NROWA + (R1 - R0) + (R0 == 0 ? 1 : 0);
+    static __host__ __device__ constexpr int la(int i, int j) { return (i - R0) * D - (i * (i - 1) - R0 * (R0 - 1)) / 2 + (j - i); }
+    static __host__ __device__ constexpr int lb(int i) { return NROWA + (i - R0); }
+    static __host__ __device__ constexpr int lc() { return NROWA + (R1 - R0); }
+    // column of local sum l in the full layout (upper triangle of dA, db, dc)
+    static __device__ int column(int l) {
+        if (R0 == 0 && l == lc()) return NA - 1;
+        if (l >= NROWA) return NT + R0 + (l - NROWA);
+        int i = R0, off = 0;
+        while (l - off >= D - i) {
+            off += D - i;
+            ++i;
+        }
+        return i * D - i * (i - 1) / 2 + (l - off);
+    }
+};
+
+// Reduce NL sums over the block (DPP within waves, waves in order) and
+// either store or add them into the block's partial row, at the group's columns.
+template <int D, int R0, int R1, bool ADD, int NL>
+__device__ __forceinline__ void block_store_group(double2 (&acc)[NL], double2 *row) {
+    using RG = RowGroup<D, R0, R1>;
+    static_assert(NL == RG::NL, "accumulator count");
+    __shared__ double2 red[(256 / MM_WAVE) * NL];
+    wave_sum_dpp<NL>(acc);
+    const int lane = threadIdx.x & (MM_WAVE - 1), w = threadIdx.x / MM_WAVE;
+    if (lane == MM_WAVE - 1) {
+#pragma unroll
+        for (int e = 0; e < NL; ++e) red[w * NL + e] = acc[e];
+    }
+    __syncthreads();
+    if (threadIdx.x < NL) {
+        double2 t = red[threadIdx.x];
+        for (int ww = 1; ww < int(blockDim.x / MM_WAVE); ++ww) t = cadd(t, red[ww * NL + threadIdx.x]);
+        const int col = RG::column(threadIdx.x);
+        row[col] = ADD ? cadd(row[col], t) : t;
+    }
+}
+
+// gradients.py:64-80 as a parallel reduction over a whole lattice held in
+// memory.  Branch-free: a term whose digit is zero reads the point itself
+// with weight sqrt(0) = 0, so every gather of a point issues together.
+template <int D, int R0, int R1>
 __global__ __launch_bounds__(256) void vjp_kernel(VjpArgs a) {
-    constexpr int NA = D * (D + 1) / 2 + D + 1;
-    constexpr int NT = D * (D + 1) / 2;
+    using RG = RowGroup<D, R0, R1>;
+    constexpr int NL = RG::NL;
     const int64_t t = blockIdx.y;
     const double2 *G = a.G + t * a.g.N;
     const double2 *gg = a.gg + t * a.g.N;
-    double2 acc[NA];
+    double2 acc[NL];
 #pragma unroll
-    for (int e = 0; e < NA; ++e) acc[e] = make_double2(0.0, 0.0);
+    for (int e = 0; e < NL; ++e) acc[e] = make_double2(0.0, 0.0);
     const int64_t f0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t inc = int64_t(gridDim.x) * blockDim.x;
     if (f0 < a.g.N) {
+        int64_t st[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) st[i] = a.g.stride[i];
         int k[D], ki[D];
         digits_of<D>(f0, a.g, 0, k);
         digits_of<D>(inc < a.g.N ? inc : 0, a.g, 0, ki);
         for (int64_t f = f0; f < a.g.N; f += inc) {
             const double2 gk = gg[f];
-            acc[NA - 1] = cfma(G[f], gk, acc[NA - 1]);
+            if (R0 == 0) acc[RG::lc()] = cfma(G[f], gk, acc[RG::lc()]);
 #pragma unroll
-            for (int i = 0; i < D; ++i) {
+            for (int i = R0; i < R1; ++i) {
                 const int kk = k[i];
-                if (kk == 0) continue;
                 const double si = a.sq[kk];
-                const int64_t p = f - a.g.stride[i];
-                acc[NT + i] = caxpy(si, cmul(G[p], gk), acc[NT + i]);
-                const int tii = i * D - i * (i - 1) / 2;
-                if (kk > 1) acc[tii] = caxpy(0.5 * si * a.sq[kk - 1], cmul(G[p - a.g.stride[i]], gk), acc[tii]);
+                const double2 ui = make_double2(si * gk.x, si * gk.y);
+                const int64_t p = f - (kk > 0 ? st[i] : 0);
+                acc[RG::lb(i)] = cfma(G[p], ui, acc[RG::lb(i)]);
+                const double h = 0.5 * a.sq[kk > 0 ? kk - 1 : 0];
+                const double2 t2 = G[p - (kk > 1 ? st[i] : 0)];
+                acc[RG::la(i, i)] = cfma(make_double2(h * t2.x, h * t2.y), ui, acc[RG::la(i, i)]);
 #pragma unroll
                 for (int j = i + 1; j < D; ++j) {
-                    if (k[j] > 0) acc[tii + (j - i)] = caxpy(si * a.sq[k[j]], cmul(G[p - a.g.stride[j]], gk), acc[tii + (j - i)]);
+                    const bool ok = kk > 0 && k[j] > 0;
+                    const double2 t3 = G[ok ? p - st[j] : f];
+                    const double w = a.sq[k[j]];
+                    acc[RG::la(i, j)] = cfma(make_double2(w * t3.x, w * t3.y), ui, acc[RG::la(i, j)]);
                 }
             }
             if (f + inc < a.g.N) digits_add<D>(k, ki, a.g, 0);
         }
     }
-    __shared__ double2 red[(256 / MM_WAVE) * NA];
-    wave_sum_many<NA>(acc);
-    const int lane = threadIdx.x & (MM_WAVE - 1), w = threadIdx.x / MM_WAVE;
-    if (lane == 0) {
-#pragma unroll
-        for (int e = 0; e < NA; ++e) red[w * NA + e] = acc[e];
-    }
-    __syncthreads();
-    if (threadIdx.x < NA) {
-        double2 s = red[threadIdx.x];
-        for (int ww = 1; ww < int(blockDim.x / MM_WAVE); ++ww) s = cadd(s, red[ww * NA + threadIdx.x]);
-        a.partial[(t * a.nchunks + blockIdx.x) * NA + threadIdx.x] = s;
-    }
+    block_store_group<D, R0, R1, false>(acc, a.partial + (t * a.nchunks + blockIdx.x) * RG::NA);
 }
+
+// VJP terms of the points of level-0 slab m (k_0 = m), the three slabs given
+// as separate buffers (the recompute window); each block adds its sums into
+// its own partial row (launches are stream-ordered, so the order of the
+// additions is fixed).  Row 0 reads slabs m-1 and m-2; with m < 1 / m < 2
+// their weights are sqrt(0) = 0 and the pointers are only dereferenced at
+// valid slab-0 addresses.
+struct SlabVjpArgs {
+    Geom g;
+    const double2 *cur, *p1, *p2, *gg;
+    double2 *partial;
+    const double *sq;
+    int m;
+};
+
+template <int D, int R0, int R1>
+__global__ __launch_bounds__(256) void vjp_slab_kernel(SlabVjpArgs a) {
+    using RG = RowGroup<D, R0, R1>;
+    constexpr int NL = RG::NL;
+    const int m = a.m;
+    const int64_t S = a.g.stride[0];
+    double2 acc[NL];
+#pragma unroll
+    for (int e = 0; e < NL; ++e) acc[e] = make_double2(0.0, 0.0);
+    const int64_t q0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t inc = int64_t(gridDim.x) * blockDim.x;
+    if (q0 < S) {
+        int64_t st[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) st[i] = a.g.stride[i];
+        int k[D], ki[D];
+        digits_of<D>(q0, a.g, 1, k);
+        digits_of<D>(inc < S ? inc : 0, a.g, 1, ki);
+        k[0] = m;
+        ki[0] = 0;
+        const double2 *p1 = m >= 1 ? a.p1 : a.cur;
+        const double2 *p2 = m >= 2 ? a.p2 : a.cur;
+        for (int64_t q = q0; q < S; q += inc) {
+            const double2 gk = a.gg[q];
+            if (R0 == 0) {
+                acc[RG::lc()] = cfma(a.cur[q], gk, acc[RG::lc()]);
+                const double s0 = a.sq[m];
+                const double2 u0 = make_double2(s0 * gk.x, s0 * gk.y);
+                acc[RG::lb(0)] = cfma(p1[q], u0, acc[RG::lb(0)]);
+                const double h = 0.5 * a.sq[m > 0 ? m - 1 : 0];
+                const double2 t2 = p2[q];
+                acc[RG::la(0, 0)] = cfma(make_double2(h * t2.x, h * t2.y), u0, acc[RG::la(0, 0)]);
+#pragma unroll
+                for (int j = 1; j < D; ++j) {
+                    const double2 t3 = p1[k[j] > 0 ? q - st[j] : q];
+                    const double w = a.sq[k[j]];
+                    acc[RG::la(0, j)] = cfma(make_double2(w * t3.x, w * t3.y), u0, acc[RG::la(0, j)]);
+                }
+            }
+#pragma unroll
+            for (int i = (R0 > 1 ? R0 : 1); i < R1; ++i) {
+                const int kk = k[i];
+                const double si = a.sq[kk];
+                const double2 ui = make_double2(si * gk.x, si * gk.y);
+                const int64_t p = q - (kk > 0 ? st[i] : 0);
+                acc[RG::lb(i)] = cfma(a.cur[p], ui, acc[RG::lb(i)]);
+                const double h = 0.5 * a.sq[kk > 0 ? kk - 1 : 0];
+                const double2 t2 = a.cur[p - (kk > 1 ? st[i] : 0)];
+                acc[RG::la(i, i)] = cfma(make_double2(h * t2.x, h * t2.y), ui, acc[RG::la(i, i)]);
+#pragma unroll
+                for (int j = i + 1; j < D; ++j) {
+                    const bool ok = kk > 0 && k[j] > 0;
+                    const double2 t3 = a.cur[ok ? p - st[j] : q];
+                    const double w = a.sq[k[j]];
+                    acc[RG::la(i, j)] = cfma(make_double2(w * t3.x, w * t3.y), ui, acc[RG::la(i, j)]);
+                }
+            }
+            if (q + inc < S) digits_add<D>(k, ki, a.g, 1);
+        }
+    }
+    block_store_group<D, R0, R1, true>(acc, a.partial + int64_t(blockIdx.x) * RG::NA);
+}
+
+// Row groups per D: one for D <= 4, two for D = 5, 6, three for D = 7, 8.
+template <int D>
+struct RowSplit {
+    static constexpr int NG = D <= 4 ? 1 : (D <= 6 ? 2 : 3);
+    static constexpr int B1 = D <= 4 ? D : 2;
+    static constexpr int B2 = D <= 6 ? D : 4;
+};
 
 // One block per triple.  npart partial rows of NA sums each (the VJP kernel's
 // chunks, optionally followed by extra rows from other kernels).

@@ -286,6 +286,23 @@ int launch_slab(const SlabArgs &a, int D, int64_t Sd, int num_cu, hipStream_t s)
 
 // ------------------------------------------------------------------ VJP path
 
+// One launch per pivot-row group (RowSplit) of the VJP kernels.
+template <int D>
+void launch_vjp_groups(const VjpArgs &a, dim3 grid, hipStream_t s) {
+    using RS = RowSplit<D>;
+    hipLaunchKernelGGL((vjp_kernel<D, 0, RS::B1>), grid, dim3(256), 0, s, a);
+    if constexpr (RS::NG >= 2) hipLaunchKernelGGL((vjp_kernel<D, RS::B1, RS::B2>), grid, dim3(256), 0, s, a);
+    if constexpr (RS::NG >= 3) hipLaunchKernelGGL((vjp_kernel<D, RS::B2, D>), grid, dim3(256), 0, s, a);
+}
+
+template <int D>
+void launch_vjp_slab_groups(const SlabVjpArgs &v, int nblk, hipStream_t s) {
+    using RS = RowSplit<D>;
+    hipLaunchKernelGGL((vjp_slab_kernel<D, 0, RS::B1>), dim3(unsigned(nblk)), dim3(256), 0, s, v);
+    if constexpr (RS::NG >= 2) hipLaunchKernelGGL((vjp_slab_kernel<D, RS::B1, RS::B2>), dim3(unsigned(nblk)), dim3(256), 0, s, v);
+    if constexpr (RS::NG >= 3) hipLaunchKernelGGL((vjp_slab_kernel<D, RS::B2, D>), dim3(unsigned(nblk)), dim3(256), 0, s, v);
+}
+
 int vjp_chunks(int64_t N, int64_t B) {
     int64_t per = 256 * 8;
     int64_t want = (N + per - 1) / per;
@@ -314,7 +331,7 @@ int launch_vjp(const Shape &S, int64_t B, const double *G, const double *g, cons
     switch (S.D) {
 #define CASE(DD)                                                                                               \
     case DD:                                                                                                   \
-        hipLaunchKernelGGL((vjp_kernel<DD>), grid, dim3(256), 0, s, a);                                       \
+        launch_vjp_groups<DD>(a, grid, s);                                                                     \
         LAUNCH_CHECK("vjp_kernel");                                                                            \
         hipLaunchKernelGGL((vjp_finalize<DD>), dim3(unsigned(B)), dim3(64), 0, s, a.partial, a.nchunks, c2, dA2, db2, dc2); \
         LAUNCH_CHECK("vjp_finalize");                                                                          \
@@ -581,7 +598,7 @@ int vjp_recompute(const Shape &S, const double *A, const double *b, const double
         v.m = m;
         switch (S.D) {
 #define CASE(DD) \
-    case DD: hipLaunchKernelGGL((vjp_slab_kernel<DD>), dim3(unsigned(nblk)), dim3(256), 0, s, v); break;
+    case DD: launch_vjp_slab_groups<DD>(v, nblk, s); break;
             CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
             default:
