@@ -201,6 +201,36 @@ def main():
         G1 = torch.empty(shape1, dtype=torch.complex128, device=dev)
         t_dm = timed(lambda: st.vanilla_hip(shape1, A1t, b1t, c1t, out=G1), 20, 2)
         extra["dm_40x4_fwd_amps_per_s"] = G1.numel() / t_dm
+        del G1
+        # ---- 4-mode lossy density matrix, shape (10,)^8 = 1e8 amplitudes:
+        #      forward, VJP of the stored G, and the VJP that recomputes G in a
+        #      three-slab window
+        log("4-mode DM (10,)^8: forward, VJP, recompute VJP")
+        (A4, b4, c4), shape4 = W.dm8_10x8()
+        A4t, b4t, c4t = (torch.as_tensor(np.asarray(x), device=dev) for x in (A4, b4, c4))
+        G4 = torch.empty(shape4, dtype=torch.complex128, device=dev)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(4)
+        g4 = torch.randn(shape4, dtype=torch.complex128, device=dev, generator=gen)
+        n4 = G4.numel()
+        t4f = timed(lambda: st.vanilla_hip(shape4, A4t, b4t, c4t, out=G4), 5, 1)
+        t4v = timed(lambda: st.vanilla_vjp_hip(G4, c4t, g4), 5, 1)
+        t4r = timed(lambda: st.vanilla_vjp_recompute_hip(shape4, A4t, b4t, c4t, g4), 5, 1)
+        r1 = st.vanilla_vjp_hip(G4, c4t, g4)
+        r2 = st.vanilla_vjp_recompute_hip(shape4, A4t, b4t, c4t, g4)
+        agree = max(float((x - y).abs().max() / x.abs().max()) for x, y in zip(r1, r2))
+        dc_identity = abs(complex(r1[2]) * complex(c4) - complex(torch.sum(G4 * g4))) / abs(complex(torch.sum(G4 * g4)))
+        extra["dm8_10x8"] = {
+            "amplitudes": n4,
+            "fwd_amps_per_s": n4 / t4f,
+            "vjp_amps_per_s": n4 / t4v,
+            "fwd_plus_vjp_amps_per_s": n4 / (t4f + t4v),
+            "recompute_vjp_amps_per_s": n4 / t4r,
+            "recompute_resident_bytes": 3 * 16 * n4 // shape4[0],
+            "stored_vs_recompute_rel_err": agree,
+            "dc_times_c_vs_sum_G_g_rel_err": dc_identity,
+        }
+        del G4, g4
 
     cpu = None
     if not args.no_cpu and not args.quick:

@@ -160,10 +160,14 @@ int kernel_dmax(int D) { return lds_kernel_dmax8(D) ? 8 : 3; }
 
 // LDS bytes of one team sweeping S from level `top` in the kernel
 // instantiation for `kdmax` (3 or 8) indices.
-int64_t lds_bytes(const Shape &S, int top, bool fused, int nwaves, int kdmax) {
+int64_t lds_bytes_rm(const Shape &S, int top, bool fused, int nwaves, int rmax) {
     int smax = 1;
     for (int j = top; j < S.D; ++j) smax = std::max(smax, int(S.s[j]));
-    return lds_layout(S.D, top, int(S.st[top]), smax, fused ? 1 : 0, nwaves, rmax_for(kdmax, fused)).total;
+    return lds_layout(S.D, top, int(S.st[top]), smax, fused ? 1 : 0, nwaves, rmax).total;
+}
+
+int64_t lds_bytes(const Shape &S, int top, bool fused, int nwaves, int kdmax) {
+    return lds_bytes_rm(S, top, fused, nwaves, rmax_for(kdmax, fused));
 }
 
 bool packed_bits_ok(const Shape &S, int top) {
@@ -226,6 +230,39 @@ int choose_top(const Shape &S, bool fused, int max_lds, int maxt, int *threads) 
 template <typename K>
 void raise_lds_limit(K kernel, int bytes) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+// Forward-only uniform batches of D <= 3 triples whose level-0 slab has
+// LEAN_MIN_S .. LEAN_MAX_S points: one point per thread of a workgroup of up
+// to 1024 threads, in lds_block_kernel (about 70 registers: four waves per
+// SIMD).  The fused sweep keeps lds_kernel: its one-point-per-thread variant
+// needs 128 registers with spills and measured slower than two points per
+// thread at two waves per SIMD.
+constexpr int LEAN_MAX_S = 1024;
+constexpr int LEAN_MIN_S = 512;
+
+// Launch lds_block_kernel for S if it applies; returns 1 if it does not.
+int launch_lean(const LdsArgs &a0, const Shape &S, bool fused, int64_t B, const DevState *ds, hipStream_t s) {
+    if (fused || S.D < 2 || S.D > 3 || S.st[0] < LEAN_MIN_S || S.st[0] > LEAN_MAX_S || !packed_bits_ok(S, 0)) return 1;
+    const int T = pick_threads(S.st[0], 1024, 1);
+    if (!T) return 1;
+    const int64_t shmem = lds_bytes_rm(S, 0, fused, T / MM_WAVE, 1);
+    if (shmem > ds->max_lds) return 1;
+    LdsArgs a = a0;
+    a.tmpl = make_job(S, 0, KIND_BLOCK);
+    const int grid = int(std::min<int64_t>(B, int64_t(1) << 30));
+#define MM_LEAN(DD, FF)                                                                                  \
+    do {                                                                                                 \
+        raise_lds_limit(lds_block_kernel<DD, FF, 1024, 1>, int(shmem));                                  \
+        hipLaunchKernelGGL((lds_block_kernel<DD, FF, 1024, 1>), dim3(grid), dim3(T), size_t(shmem), s, a); \
+    } while (0)
+    if (S.D == 2)
+        MM_LEAN(2, false);
+    else
+        MM_LEAN(3, false);
+#undef MM_LEAN
+    LAUNCH_CHECK("lds_block_kernel");
+    return MM_OK;
 }
 
 // Launch the LDS kernel instantiation that covers D / fused.
@@ -438,6 +475,10 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
         a.tmpl = make_job(S, 0, KIND_LANE);
         int grid = int((B + 63) / 64);
         return launch_lds(a, 1, fused, grid, 64, 0, s);
+    }
+    {
+        const int rc = launch_lean(a, S, fused, B, ds, s);
+        if (rc <= 0) return rc;
     }
     int T = 64;
     const int maxt = lds_maxt(S.D, fused);

@@ -125,4 +125,23 @@ __global__ __launch_bounds__(MAXT) void lds_kernel(LdsArgs a) {
     }
 }
 
+// Uniform batches whose triples all take one workgroup each, with D known at
+// compile time and RM points per thread: nothing but that one sweep is in
+// the kernel, so it needs fewer registers than lds_kernel and a 1024-thread
+// workgroup keeps four waves on every SIMD.
+template <int D, bool FUSED, int MAXT, int RM>
+__global__ __launch_bounds__(MAXT) void lds_block_kernel(LdsArgs a) {
+    extern __shared__ __align__(16) char lds[];
+    JobShift sh;
+    sh.ua = a.ua;
+    sh.ub = a.ub;
+    sh.uc = a.uc;
+    sh.ug = a.ug;
+    for (int64_t k = blockIdx.x; k < a.n; k += gridDim.x) {
+        sh.k = k;
+        load_job<false>(lds, &a.tmpl, sh);
+        sweep_lds<D, FUSED, false, RM>(a.P, a.T, lds);
+    }
+}
+
 }  // namespace mm

@@ -55,6 +55,7 @@ FWD_SHAPES = [
     (4, 80, 80),        # axis-0 slab too large for LDS: sub-lattice in LDS, slabs in global memory
     (2, 3, 40, 45),
     (2, 4, 4, 4, 4, 4, 4, 4),
+    (5, 700), (4, 20, 40), (3, 32, 32),   # level-0 slab of 512..1024 points: one point per thread
 ]
 
 
@@ -120,7 +121,8 @@ def test_vjp(st, oracle_mod, shape):
         assert rel(np_(x), rs) <= TOL
 
 
-@pytest.mark.parametrize("shape", [(23,), (9, 11), (30, 30, 30), (5, 4, 3, 6), (4, 80, 80)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("shape", [(23,), (9, 11), (30, 30, 30), (5, 4, 3, 6), (4, 80, 80), (6, 700), (5, 20, 40)],
+                         ids=lambda s: "x".join(map(str, s)))
 def test_fused(st, oracle_mod, shape):
     rng = np.random.default_rng(sum(shape) + 2000)
     A, b, c = triple(rng, len(shape))
@@ -133,7 +135,7 @@ def test_fused(st, oracle_mod, shape):
         assert rel(np_(x), r) <= TOL
 
 
-@pytest.mark.parametrize("shape", [(12,), (20, 30), (7, 6, 5), (30, 30, 30)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("shape", [(12,), (20, 30), (7, 6, 5), (30, 30, 30), (9, 16, 32), (4, 600)], ids=lambda s: "x".join(map(str, s)))
 def test_uniform_batch(st, oracle_mod, shape):
     rng = np.random.default_rng(sum(shape) + 3000)
     B = 7
