@@ -220,6 +220,26 @@ def main():
         r2 = st.vanilla_vjp_recompute_hip(shape4, A4t, b4t, c4t, g4)
         agree = max(float((x - y).abs().max() / x.abs().max()) for x, y in zip(r1, r2))
         dc_identity = abs(complex(r1[2]) * complex(c4) - complex(torch.sum(G4 * g4))) / abs(complex(torch.sum(G4 * g4)))
+        del g4, r1, r2
+        # contraction of the density matrix, (bra) x (ket) = 1e4 x 1e4, with a
+        # projector P = V V^H onto a random 16-dimensional subspace of the
+        # 4-mode number basis: P @ rho on the FP64 matrix cores
+        from mrmustard_amd.contraction import dm_operator_product, zgemm_hip
+
+        log("4-mode DM (10,)^8: projector contraction on MFMA")
+        dmat = 10**4
+        V = torch.randn(dmat, 16, dtype=torch.complex128, device=dev, generator=gen)
+        for j in range(16):  # Gram-Schmidt: orthonormal columns
+            for i in range(j):
+                V[:, j] -= torch.vdot(V[:, i], V[:, j]) * V[:, i]
+            V[:, j] /= torch.linalg.vector_norm(V[:, j])
+        P = zgemm_hip(V, V.conj().T.contiguous())
+        Cp = torch.empty_like(G4)
+        t4c = timed(lambda: dm_operator_product(P, G4, out=Cp), 3, 1)
+        rows = slice(0, 64)
+        ref_rows = torch.matmul(P[rows], G4.reshape(dmat, dmat))
+        c_err = float((Cp.reshape(dmat, dmat)[rows] - ref_rows).abs().max() / ref_rows.abs().max())
+        zflops = 8.0 * dmat**3
         extra["dm8_10x8"] = {
             "amplitudes": n4,
             "fwd_amps_per_s": n4 / t4f,
@@ -229,8 +249,15 @@ def main():
             "recompute_resident_bytes": 3 * 16 * n4 // shape4[0],
             "stored_vs_recompute_rel_err": agree,
             "dc_times_c_vs_sum_G_g_rel_err": dc_identity,
+            "contraction": {
+                "op": "P @ rho, complex128 10^4 x 10^4 x 10^4 (zgemm_mfma_kernel)",
+                "ms": t4c * 1e3,
+                "tflops": zflops / t4c / 1e12,
+                "mfma_fp64_frac": zflops / t4c / 1e12 / FP64_PEAK_TFLOPS,
+                "rel_err_first_64_rows_vs_torch_matmul": c_err,
+            },
         }
-        del G4, g4
+        del G4, Cp, P, V
 
     cpu = None
     if not args.no_cpu and not args.quick:

@@ -107,6 +107,19 @@ int mm_hermite_vjp_recompute(const double *A, const double *b, const double *c, 
                              const int64_t *shape, int D, double *dA, double *db, double *dc,
                              void *workspace, size_t ws_bytes, void *stream);
 
+/* ------------------------------------------------------ contraction (MFMA)
+ *
+ * C = A * B for complex128 row-major matrices (A: M x K with row stride lda,
+ * B: K x N, ldb, C: M x N, ldc; strides in complex elements), on the FP64
+ * matrix cores.  The consumer of the amplitudes: a Fock-basis tensor reshaped
+ * to a matrix and contracted with another (ArrayAnsatz.contract ->
+ * BackendManager.einsum, mrmustard/physics/ansatz/array_ansatz.py:159+,
+ * mrmustard/math/backend_manager.py:526-548).  Sums over K in chunks of 16
+ * with the instruction's own accumulation order, so results differ from a
+ * sequential sum at rounding level. */
+int mm_zgemm(const double *A, int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, int64_t M, int64_t N,
+             int64_t K, void *stream);
+
 /* ------------------------------------------------- work-queue batch (plans)
  *
  * A plan describes n triples of uneven mode count D_k <= 8 and shape_k.

@@ -21,6 +21,7 @@
 #include "global_kernels.hpp"
 #include "jobs.hpp"
 #include "lds_kernel.hpp"
+#include "zgemm.hpp"
 
 using namespace mm;
 
@@ -1073,6 +1074,29 @@ int mm_hermite_vjp_recompute(const double *A, const double *b, const double *c, 
     rc = device_state(&ds);
     if (rc) return rc;
     return vjp_recompute(S, A, b, c, dLdG, dA, db, dc, workspace, ws_bytes, ds, reinterpret_cast<hipStream_t>(stream));
+}
+
+int mm_zgemm(const double *A, int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, int64_t M, int64_t N, int64_t K,
+             void *stream) {
+    if (M < 0 || N < 0 || K < 0) return fail(MM_ERR_INVALID, "negative matrix size");
+    if (M == 0 || N == 0) return MM_OK;
+    if (!A || !B || !C) return fail(MM_ERR_INVALID, "NULL matrix pointer");
+    if (lda < K || ldb < N || ldc < N) return fail(MM_ERR_INVALID, "leading dimension smaller than the row length");
+    const int64_t gx = (N + ZG_BN - 1) / ZG_BN, gy = (M + ZG_BM - 1) / ZG_BM;
+    if (gx > 0x7fffffff || gy > 65535) return fail(MM_ERR_UNSUPPORTED, "matrix too large for one launch");
+    ZgemmArgs a;
+    a.A = reinterpret_cast<const double2 *>(A);
+    a.B = reinterpret_cast<const double2 *>(B);
+    a.C = reinterpret_cast<double2 *>(C);
+    a.M = M;
+    a.N = N;
+    a.K = K;
+    a.lda = lda;
+    a.ldb = ldb;
+    a.ldc = ldc;
+    hipLaunchKernelGGL(zgemm_mfma_kernel, dim3(unsigned(gx), unsigned(gy)), dim3(ZG_THREADS), 0, reinterpret_cast<hipStream_t>(stream), a);
+    LAUNCH_CHECK("zgemm_mfma_kernel");
+    return MM_OK;
 }
 
 }  // extern "C"

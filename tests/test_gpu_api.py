@@ -274,3 +274,64 @@ def test_shim_backend_numbers(oracle_mod):
     Ab, bb, cb = random_triple(2, (3,), seed=22)
     Gb = m._backend.hermite_renormalized_batched(Ab, bb, cb, (4, 5))
     assert np.array_equal(Gb, oracle_mod.vanilla_batch((4, 5), Ab, bb, cb))
+
+
+# ---------------------------------------------------------------- contraction
+
+
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (16, 16, 4), (64, 128, 16), (37, 150, 23), (130, 70, 200), (5, 3, 1000)],
+                         ids=lambda t: "x".join(map(str, t)))
+def test_zgemm(mnk):
+    """mm_zgemm against numpy's complex matmul (the instruction sums K in
+    blocks, so the bar is normwise 1e-13 relative)."""
+    import torch
+
+    from mrmustard_amd.contraction import zgemm_hip
+
+    M, N, K = mnk
+    rng = np.random.default_rng(M * 1000 + N * 10 + K)
+    A = rng.normal(size=(M, K)) + 1j * rng.normal(size=(M, K))
+    B = rng.normal(size=(K, N)) + 1j * rng.normal(size=(K, N))
+    print(f"zgemm {mnk}", flush=True)
+    C = zgemm_hip(A, B).cpu().numpy()
+    ref = A @ B
+    err = np.max(np.abs(C - ref)) / np.max(np.abs(ref))
+    assert err < 1e-13, err
+    # a real A and an imaginary B separate the four real products
+    Ar, Bi = A.real + 0j, 1j * B.imag
+    C2 = zgemm_hip(Ar, Bi).cpu().numpy()
+    assert np.max(np.abs(C2 - Ar @ Bi)) <= 1e-13 * np.max(np.abs(Ar @ Bi))
+    C3 = zgemm_hip(torch.tensor(A, device="cuda").T.contiguous(), torch.tensor(A, device="cuda")).cpu().numpy()
+    assert np.max(np.abs(C3 - A.T @ A)) <= 1e-13 * np.max(np.abs(A.T @ A))
+
+
+def test_dm_operator_product():
+    """The product of an operator with a lossy 2-mode density matrix from the
+    path: trace of P rho for a number-basis projector equals the sum of the
+    projected populations."""
+    from mrmustard_amd.contraction import dm_operator_product
+    from mrmustard_amd.lattice import strategies as st
+    from mrmustard_amd import workloads as W
+
+    (A, b, c), _ = W.lossy_dm_40x4()
+    shape = (8, 8, 8, 8)
+    rho = st.vanilla_hip(shape, A, b, c)
+    d = 64
+    keep = np.zeros(d)
+    keep[np.random.default_rng(5).choice(d, 20, replace=False)] = 1.0
+    P = np.diag(keep).astype(complex)
+    out = dm_operator_product(P, rho)
+    assert out.shape == rho.shape
+    r = rho.reshape(d, d).cpu().numpy()
+    got = out.reshape(d, d).cpu().numpy()
+    assert np.max(np.abs(got - P @ r)) <= 1e-13 * np.max(np.abs(r))
+    assert abs(np.trace(got) - np.sum(keep * np.diag(r))) <= 1e-13 * np.max(np.abs(r))
+
+
+def test_zgemm_errors():
+    from mrmustard_amd.contraction import zgemm_hip
+
+    with pytest.raises(ValueError):
+        zgemm_hip(np.ones((3, 4), complex), np.ones((5, 2), complex))
+    with pytest.raises(ValueError):
+        zgemm_hip(np.ones(3, complex), np.ones((3, 2), complex))
