@@ -325,19 +325,26 @@ struct RowSplit {
 };
 
 // One block per triple.  npart partial rows of NA sums each (the VJP kernel's
-// chunks, optionally followed by extra rows from other kernels).
+// chunks, optionally followed by extra rows from other kernels).  Wave w sums
+// columns w, w + nwaves, ...: lane l adds rows l, l + 64, ... in order, then
+// the lanes combine in a fixed butterfly, so the result does not depend on
+// timing.
 template <int D>
-__global__ __launch_bounds__(64) void vjp_finalize(const double2 *partial, int npart, const double2 *c, double2 *dA, double2 *db,
-                                                   double2 *dc) {
+__global__ __launch_bounds__(1024) void vjp_finalize(const double2 *partial, int npart, const double2 *c, double2 *dA, double2 *db,
+                                                     double2 *dc) {
     constexpr int NA = D * (D + 1) / 2 + D + 1;
     constexpr int NT = D * (D + 1) / 2;
     __shared__ double2 s[NA];
     const int64_t t = blockIdx.x;
     const double2 *p = partial + t * npart * NA;
-    if (threadIdx.x < NA) {
+    const int lane = threadIdx.x & (MM_WAVE - 1);
+    const int nw = blockDim.x / MM_WAVE;
+    for (int e = threadIdx.x / MM_WAVE; e < NA; e += nw) {
         double2 acc = make_double2(0.0, 0.0);
-        for (int r = 0; r < npart; ++r) acc = cadd(acc, p[r * NA + threadIdx.x]);
-        s[threadIdx.x] = acc;
+#pragma unroll 8
+        for (int r = lane; r < npart; r += MM_WAVE) acc = cadd(acc, p[int64_t(r) * NA + e]);
+        acc = wave_sum(acc);
+        if (lane == 0) s[e] = acc;
     }
     __syncthreads();
     if (threadIdx.x < D * D) {

@@ -341,6 +341,10 @@ void launch_vjp_slab_groups(const SlabVjpArgs &v, int nblk, hipStream_t s) {
     if constexpr (RS::NG >= 3) hipLaunchKernelGGL((vjp_slab_kernel<D, RS::B2, D>), dim3(unsigned(nblk)), dim3(256), 0, s, v);
 }
 
+// Threads of the finalize block of each triple: one wave per sum column for
+// few triples, four waves when many triples fill the GPU anyway.
+int finalize_threads(int64_t B) { return B >= 64 ? 256 : 1024; }
+
 int vjp_chunks(int64_t N, int64_t B) {
     int64_t per = 256 * 8;
     int64_t want = (N + per - 1) / per;
@@ -371,7 +375,7 @@ int launch_vjp(const Shape &S, int64_t B, const double *G, const double *g, cons
     case DD:                                                                                                   \
         launch_vjp_groups<DD>(a, grid, s);                                                                     \
         LAUNCH_CHECK("vjp_kernel");                                                                            \
-        hipLaunchKernelGGL((vjp_finalize<DD>), dim3(unsigned(B)), dim3(64), 0, s, a.partial, a.nchunks, c2, dA2, db2, dc2); \
+        hipLaunchKernelGGL((vjp_finalize<DD>), dim3(unsigned(B)), dim3(finalize_threads(B)), 0, s, a.partial, a.nchunks, c2, dA2, db2, dc2); \
         LAUNCH_CHECK("vjp_finalize");                                                                          \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
@@ -653,7 +657,7 @@ int vjp_recompute(const Shape &S, const double *A, const double *b, const double
     double2 *dA2 = reinterpret_cast<double2 *>(dA), *db2 = reinterpret_cast<double2 *>(db), *dc2 = reinterpret_cast<double2 *>(dc);
     switch (S.D) {
 #define CASE(DD) \
-    case DD: hipLaunchKernelGGL((vjp_finalize<DD>), dim3(1), dim3(64), 0, s, partial, nblk, c2, dA2, db2, dc2); break;
+    case DD: hipLaunchKernelGGL((vjp_finalize<DD>), dim3(1), dim3(finalize_threads(1)), 0, s, partial, nblk, c2, dA2, db2, dc2); break;
         CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
         default:

@@ -233,6 +233,35 @@ def test_shape_and_d_errors(math):
         math.hermite_renormalized(A9, b9, c9, (2,) * 9)  # D > 8
 
 
+def test_empty_batches(math, oracle_mod):
+    """Zero triples: every batched entry point returns an empty result without
+    launching (test_vanilla.py:188-238 use B >= 1; B = 0 is the edge)."""
+    from mrmustard_amd.lattice import strategies as st
+
+    A = np.zeros((0, 2, 2), complex)
+    b = np.zeros((0, 2), complex)
+    c = np.zeros((0,), complex)
+    G = math.hermite_renormalized(A, b, c, (3, 4))
+    assert tuple(G.shape) == (0, 3, 4)
+    assert tuple(st.vanilla_batch_hip((3, 4), A, b, c).shape) == (0, 3, 4)
+    g = np.zeros((0, 3, 4), complex)
+    Gf, dA, db, dc = st.vanilla_batch_fused_hip((3, 4), A, b, c, g)
+    assert tuple(Gf.shape) == (0, 3, 4) and tuple(dA.shape) == (0, 2, 2) and tuple(db.shape) == (0, 2) and tuple(dc.shape) == (0,)
+    dA, db, dc = st.vanilla_batch_vjp_hip(np.zeros((0, 3, 4), complex), c, g)
+    assert tuple(dA.shape) == (0, 2, 2)
+
+
+def test_largest_cutoff(oracle_mod):
+    """A one-index chain up to the end of the sqrt table (cutoff 99999)."""
+    from mrmustard_amd.lattice import strategies as st
+
+    A = np.array([[0.3 + 0.1j]])
+    b = np.array([0.2 - 0.4j])
+    c = 1.0 + 0.0j
+    G = st.vanilla_hip((99999,), A, b, c).cpu().numpy()
+    assert np.array_equal(G, oracle_mod.vanilla((99999,), A, b, c))
+
+
 def test_workspace_errors_surface_as_exceptions(math):
     """A C-ABI call with a missing workspace fails with a clear error."""
     import ctypes
