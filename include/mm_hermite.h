@@ -107,6 +107,19 @@ int mm_hermite_vjp_recompute(const double *A, const double *b, const double *c, 
                              const int64_t *shape, int D, double *dA, double *db, double *dc,
                              void *workspace, size_t ws_bytes, void *stream);
 
+/* ---------------------------------------------------------------- autoshape
+ *
+ * Fock cutoffs of B Gaussian density matrices of M <= 8 modes: for every mode
+ * the least k whose diagonal probability sum reaches max_prob (early-stopped
+ * diagonal recursion of the single-mode reduced state), clipped to
+ * [min_shape, max_shape].  Replaces autoshape_numba
+ * (mrmustard/math/lattice/autoshape.py:23-154), called by State.auto_shape
+ * (mrmustard/lab/states/base.py:427-434).  A: B x (2M x 2M), indices
+ * (bra modes, ket modes); b: B x 2M; c: B; shape: B x M int64 (device).  A
+ * mode whose complement matrix is singular gets -1 (the reference raises). */
+int mm_autoshape(const double *A, const double *b, const double *c, int64_t B, int M, double max_prob, int max_shape,
+                 int min_shape, int64_t *shape, void *stream);
+
 /* ------------------------------------------------------ contraction (MFMA)
  *
  * C = A * B for complex128 row-major matrices (A: M x K with row stride lda,

@@ -22,6 +22,7 @@
 #include "jobs.hpp"
 #include "lds_kernel.hpp"
 #include "zgemm.hpp"
+#include "autoshape.hpp"
 
 using namespace mm;
 
@@ -1100,6 +1101,34 @@ int mm_zgemm(const double *A, int64_t lda, const double *B, int64_t ldb, double 
     a.ldc = ldc;
     hipLaunchKernelGGL(zgemm_mfma_kernel, dim3(unsigned(gx), unsigned(gy)), dim3(ZG_THREADS), 0, reinterpret_cast<hipStream_t>(stream), a);
     LAUNCH_CHECK("zgemm_mfma_kernel");
+    return MM_OK;
+}
+
+int mm_autoshape(const double *A, const double *b, const double *c, int64_t B, int M, double max_prob, int max_shape,
+                 int min_shape, int64_t *shape, void *stream) {
+    if (M < 1 || M > AS_MAXM) return fail(MM_ERR_INVALID, "autoshape needs 1 <= M <= %d modes, got %d", AS_MAXM, M);
+    if (B < 0) return fail(MM_ERR_INVALID, "negative batch size");
+    if (max_shape < 1 || int64_t(max_shape) + 2 >= MM_MAX_CUTOFF)
+        return fail(MM_ERR_INVALID, "max_shape must be in [1, %d)", MM_MAX_CUTOFF - 2);
+    if (B == 0) return MM_OK;
+    if (!A || !b || !c || !shape) return fail(MM_ERR_INVALID, "NULL array pointer");
+    if (B * M > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "too many states for one launch");
+    DevState *ds = nullptr;
+    int rc = device_state(&ds);
+    if (rc) return rc;
+    AutoshapeArgs a;
+    a.A = reinterpret_cast<const double2 *>(A);
+    a.b = reinterpret_cast<const double2 *>(b);
+    a.c = reinterpret_cast<const double2 *>(c);
+    a.B = B;
+    a.M = M;
+    a.max_prob = max_prob;
+    a.max_shape = max_shape;
+    a.min_shape = min_shape;
+    a.sq = ds->sq;
+    a.shape = shape;
+    hipLaunchKernelGGL(autoshape_kernel, dim3(unsigned(B * M)), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), a);
+    LAUNCH_CHECK("autoshape_kernel");
     return MM_OK;
 }
 

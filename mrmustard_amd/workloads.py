@@ -163,3 +163,30 @@ def squeezed_interferometer_dm(m: int = 4, seed: int = 4):
 def dm8_10x8(seed: int = 4):
     """4-mode lossy squeezed-interferometer density matrix, shape (10,)*8: 1e8 amplitudes."""
     return squeezed_interferometer_dm(4, seed), (10,) * 8
+
+
+def normalize_dm(A, b, c):
+    """Scale c so that the density matrix has unit trace:
+    Tr = c exp(-b^T (A - X)^-1 b / 2) / sqrt(det(A - X)), X = [[0, I], [I, 0]]
+    (the Gaussian integral over all modes, autoshape.py:105-108 applied to all of them)."""
+    A = np.asarray(A, np.complex128)
+    b = np.asarray(b, np.complex128)
+    M = len(b) // 2
+    X = np.block([[np.zeros((M, M)), np.eye(M)], [np.eye(M), np.zeros((M, M))]])
+    R = A - X
+    tr = complex(c) * np.exp(-0.5 * b @ np.linalg.solve(R, b)) / np.sqrt(np.linalg.det(R))
+    return A, b, np.complex128(complex(c) / abs(tr))
+
+
+def autoshape_states(n: int = 4096, m: int = 4, seed: int = 5):
+    """n normalized lossy Gaussian density matrices of m modes (A: (n, 2m, 2m),
+    b: (n, 2m), c: (n,)), the input of a batched autoshape."""
+    rng = np.random.default_rng(seed)
+    As, bs, cs = [], [], []
+    for _ in range(n):
+        A, b, c = gaussian_ket(m, rng, r_max=0.8, b_sigma=0.6)
+        A, b, c = normalize_dm(*apply_loss(*ket_to_dm(A, b, c), rng.uniform(0.6, 1.0, m)))
+        As.append(A)
+        bs.append(b)
+        cs.append(c)
+    return np.stack(As), np.stack(bs), np.array(cs)
