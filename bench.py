@@ -250,6 +250,12 @@ def main():
             },
         }
         del G4, Cp, P, V
+        # ---- binomial strategy (fill by weight, L2 stop) on one (30,30,30) ket
+        log("binomial: one (30,30,30) ket, max_l2 = 0.99999")
+        A0, b0, c0 = A[0], b[0], c[0]
+        t_bin = timed(lambda: st.binomial_hip(shape, A0, b0, c0, 0.99999, sum(shape) - len(shape) + 1), 20, 2)
+        extra["binomial_30x30x30"] = {"amps_per_s": N / t_bin, "us_per_call": t_bin * 1e6,
+                                      "note": "one triple per launch; includes the copy of the final norm to the host"}
         # ---- autoshape of 4,096 four-mode lossy Gaussian density matrices
         log("autoshape: 4096 four-mode density matrices")
         from mrmustard_amd.lattice.autoshape import autoshape_batch_hip

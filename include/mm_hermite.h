@@ -107,6 +107,22 @@ int mm_hermite_vjp_recompute(const double *A, const double *b, const double *c, 
                              const int64_t *shape, int D, double *dA, double *db, double *dc,
                              void *workspace, size_t ws_bytes, void *stream);
 
+/* ----------------------------------------------------------------- binomial
+ *
+ * The lattice filled by total photon number w = sum_j k_j, stopping after the
+ * first w at which |c|^2 + the squared norms of weights 1..w exceed max_l2, or
+ * at w = global_cutoff - 1; entries of larger weight are zero.  Values equal
+ * vanilla's.  Replaces binomial (mrmustard/math/lattice/strategies/binomial.py
+ * :30-74) as bound by BackendNumpy.hermite_renormalized_binomial
+ * (math/backend_numpy.py:405-421).  norm (device, 1 double, may be NULL)
+ * receives the accumulated squared norm, binomial's second result.  The sum of
+ * (shape - 1) + 1 must be at most 8192.  The squared norms of a weight are
+ * summed in parallel (as numba's prange does), so a norm within rounding of
+ * max_l2 can stop one weight apart from a sequential sum. */
+size_t mm_hermite_binomial_workspace(const int64_t *shape, int D);
+int mm_hermite_binomial(const double *A, const double *b, const double *c, const int64_t *shape, int D, double max_l2,
+                        int64_t global_cutoff, double *G, double *norm, void *workspace, size_t ws_bytes, void *stream);
+
 /* ---------------------------------------------------------------- autoshape
  *
  * Fock cutoffs of B Gaussian density matrices of M <= 8 modes: for every mode

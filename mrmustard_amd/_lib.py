@@ -44,6 +44,8 @@ SIGNATURES = {
     "mm_hermite_batch_fused": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _int, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mm_hermite_vjp_recompute_workspace": (_sz, [_vp, _int]),
     "mm_hermite_vjp_recompute": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mm_hermite_binomial_workspace": (_sz, [_vp, _int]),
+    "mm_hermite_binomial": (_int, [_vp, _vp, _vp, _vp, _int, ctypes.c_double, _i64, _vp, _vp, _vp, _sz, _vp]),
     "mm_zgemm": (_int, [_vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
     "mm_autoshape": (_int, [_vp, _vp, _vp, _i64, _int, ctypes.c_double, _int, _int, _vp, _vp]),
     "mm_plan_create": (_int, [_i64, _vp, _vp, _int, _vp, _vp]),

@@ -23,6 +23,7 @@
 #include "lds_kernel.hpp"
 #include "zgemm.hpp"
 #include "autoshape.hpp"
+#include "binomial.hpp"
 
 using namespace mm;
 
@@ -670,6 +671,35 @@ int vjp_recompute(const Shape &S, const double *A, const double *b, const double
 
 // ------------------------------------------------------------------- plans
 
+// ------------------------------------------------------------ binomial path
+
+int binom_W(const Shape &S) {
+    int64_t w = 1;
+    for (int i = 0; i < S.D; ++i) w += S.s[i] - 1;
+    return int(std::min<int64_t>(w, int64_t(1) << 30));
+}
+
+int binom_blocks(const Shape &S) {
+    const int64_t per = 256 * 16;
+    return int(std::max<int64_t>(1, std::min<int64_t>((S.N + per - 1) / per, 1024)));
+}
+
+size_t binom_ws_bytes(const Shape &S) { return size_t(binom_blocks(S)) * size_t(binom_W(S)) * 8 + 256; }
+
+template <int D>
+void launch_binomial_kernels(const Shape &S, const double2 *c, double max_l2, int64_t global_cutoff, double2 *G, double *norm_out,
+                             void *ws, int num_cu, hipStream_t s) {
+    const Geom g = geom_of(S);
+    const int W = binom_W(S), nblk = binom_blocks(S);
+    const int64_t chunk = (S.N + nblk - 1) / nblk;
+    double *part = reinterpret_cast<double *>(ws);
+    int *wstop = reinterpret_cast<int *>(reinterpret_cast<char *>(ws) + size_t(nblk) * W * 8);
+    hipLaunchKernelGGL((weight_norm_kernel<D>), dim3(nblk), dim3(256), size_t(W) * 8, s, G, g, W, chunk, part);
+    hipLaunchKernelGGL(binomial_stop_kernel, dim3(1), dim3(256), size_t(W) * 8, s, part, nblk, W, c, max_l2, global_cutoff, wstop, norm_out);
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((S.N + 255) / 256, int64_t(num_cu) * 16));
+    hipLaunchKernelGGL((weight_mask_kernel<D>), dim3(unsigned(grid)), dim3(256), 0, s, G, g, wstop);
+}
+
 struct GlobalJob {
     Shape S;
     int64_t a_off, b_off, c_off, g_off;
@@ -1079,6 +1109,42 @@ int mm_hermite_vjp_recompute(const double *A, const double *b, const double *c, 
     rc = device_state(&ds);
     if (rc) return rc;
     return vjp_recompute(S, A, b, c, dLdG, dA, db, dc, workspace, ws_bytes, ds, reinterpret_cast<hipStream_t>(stream));
+}
+
+size_t mm_hermite_binomial_workspace(const int64_t *shape, int D) {
+    Shape S;
+    if (make_shape(shape, D, &S)) return 0;
+    return binom_ws_bytes(S);
+}
+
+int mm_hermite_binomial(const double *A, const double *b, const double *c, const int64_t *shape, int D, double max_l2,
+                        int64_t global_cutoff, double *G, double *norm, void *workspace, size_t ws_bytes, void *stream) {
+    Shape S;
+    int rc = make_shape(shape, D, &S);
+    if (rc) return rc;
+    if (!A || !b || !c || !G) return fail(MM_ERR_INVALID, "NULL array pointer");
+    if (global_cutoff < 1) return fail(MM_ERR_INVALID, "global_cutoff must be >= 1, got %lld", (long long)global_cutoff);
+    if (binom_W(S) > BINOM_MAX_W) return fail(MM_ERR_UNSUPPORTED, "more than %d weights (sum of shape - D + 1)", BINOM_MAX_W);
+    if (!workspace || ws_bytes < binom_ws_bytes(S))
+        return fail(MM_ERR_WORKSPACE, "binomial workspace of %zu bytes needed, %zu given", binom_ws_bytes(S), ws_bytes);
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    rc = vanilla_uniform(S, 1, A, b, c, nullptr, G, nullptr, nullptr, nullptr, nullptr, 0, ds, s);
+    if (rc) return rc;
+    const double2 *c2 = reinterpret_cast<const double2 *>(c);
+    double2 *G2 = reinterpret_cast<double2 *>(G);
+    switch (S.D) {
+#define CASE(DD) \
+    case DD: launch_binomial_kernels<DD>(S, c2, max_l2, global_cutoff, G2, norm, workspace, ds->num_cu, s); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad D");
+    }
+    LAUNCH_CHECK("binomial kernels");
+    return MM_OK;
 }
 
 int mm_zgemm(const double *A, int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, int64_t M, int64_t N, int64_t K,

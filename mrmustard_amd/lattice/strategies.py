@@ -14,6 +14,8 @@ ABI of libmmhip.so on the current stream:
   equals the two calls in sequence).
 - ``vanilla_vjp_recompute_hip``: the VJP from (A, b, c) without G, recomputing
   the lattice slab by slab in a bounded window.
+- ``binomial_hip``           <- binomial.py:30-74  ``binomial`` (fill by weight,
+  stop on the squared norm).
 
 The VJPs use the reference's holomorphic convention: ``dLdG`` is not
 conjugated, and dA is returned symmetrised, (M + M^T)/2.
@@ -30,6 +32,7 @@ import torch
 from .. import _lib
 
 __all__ = [
+    "binomial_hip",
     "stable_hip",
     "vanilla_vjp_recompute_hip",
     "vanilla_batch_fused_hip",
@@ -257,3 +260,27 @@ def vanilla_vjp_recompute_hip(shape, A, b, c, dLdG):
     )
     _lib.check(rc, "mm_hermite_vjp_recompute")
     return dA, db, dc
+
+
+def binomial_hip(local_cutoffs, A, b, c, max_l2, global_cutoff):
+    """``binomial`` (binomial.py:30-74): (G, norm).  G has shape
+    ``local_cutoffs`` and holds the amplitudes of total photon number up to the
+    first weight at which |c|^2 plus the squared norms of weights 1..w exceed
+    ``max_l2`` (None: never), or up to ``global_cutoff - 1``; zeros beyond.
+    ``norm`` is that accumulated squared norm (a float)."""
+    shape = tuple(int(s) for s in local_cutoffs)
+    A, b, c = as_device_c128(A), as_device_c128(b), as_device_c128(c)
+    D = len(shape)
+    if tuple(A.shape) != (D, D) or b.numel() != D or c.numel() != 1:
+        raise ValueError(f"A must be ({D}, {D}), b ({D},) and c scalar for shape {shape}")
+    G = torch.empty(shape, dtype=C128, device=A.device)
+    norm = torch.empty((), dtype=torch.float64, device=A.device)
+    s = _shape_arr(shape)
+    L = _lib.load()
+    ws = _workspace(L.mm_hermite_binomial_workspace(s.ctypes.data, D))
+    rc = L.mm_hermite_binomial(
+        _ptr(A), _ptr(b), _ptr(c), s.ctypes.data, D, float("inf") if max_l2 is None else float(max_l2), int(global_cutoff),
+        _ptr(G), _ptr(norm), _ptr(ws), 0 if ws is None else ws.numel(), _stream(),
+    )
+    _lib.check(rc, "mm_hermite_binomial")
+    return G, float(norm)

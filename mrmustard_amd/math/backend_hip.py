@@ -106,6 +106,14 @@ class BackendHip:
         G = (st.stable_hip if stable else st.vanilla_hip)(shape, A, b, c, out=dev_out)
         return self._deliver(G, out)
 
+    def hermite_renormalized_binomial(self, A, B, C, shape: tuple[int], max_l2, global_cutoff):
+        """Fill by weight with an L2 stop (backend_numpy.py:405-421 -> binomial.py:30-74)."""
+        return st.binomial_hip(
+            tuple(shape), A, B, C,
+            max_l2=max_l2 or settings.AUTOSHAPE_PROBABILITY,
+            global_cutoff=global_cutoff or sum(shape) - len(shape) + 1,
+        )[0]
+
     def hermite_renormalized_batched(self, A, b, c, shape: tuple[int], stable: bool = False, out=None):
         """Batched path (backend_numpy.py:394-403 -> batch.py:27)."""
         if _needs_grad(A, b, c):

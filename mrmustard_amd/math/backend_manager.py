@@ -151,6 +151,13 @@ class BackendManager:
         check_out_shape(())
         return self._apply("hermite_renormalized", (A, b, c), {"shape": tuple(shape), "stable": stable, "out": out})
 
+    def hermite_renormalized_binomial(self, A, B, C, shape: tuple[int], max_l2, global_cutoff):
+        r"""The amplitudes of ``hermite_renormalized`` filled by total photon
+        number, up to the weight at which the squared L2 norm exceeds ``max_l2``
+        (default ``settings.AUTOSHAPE_PROBABILITY``) or up to ``global_cutoff``
+        (default ``sum(shape) - len(shape) + 1``) (backend_manager.py:775-800)."""
+        return self._apply("hermite_renormalized_binomial", (A, B, C, shape, max_l2, global_cutoff))
+
     def _as_input(self, x):
         if hasattr(x, "ndim") and hasattr(x, "shape"):
             return x
