@@ -136,6 +136,26 @@ int mm_hermite_binomial(const double *A, const double *b, const double *c, const
 int mm_autoshape(const double *A, const double *b, const double *c, int64_t B, int M, double max_prob, int max_shape,
                  int min_shape, int64_t *shape, void *stream);
 
+/* ----------------------------------------------------------------- diagonal
+ *
+ * The photon-number diagonal G[k_0, k_0, k_1, k_1, ...] of the 2M-index
+ * lattice of (A, B, c) for M <= 10 modes, without the full lattice.  Replaces
+ * hermite_multidimensional_diagonal (mrmustard/math/lattice/strategies/
+ * compactFock/inputValidation.py:61-79 -> diagonal_amps.py:200-248) as bound
+ * by BackendNumpy.hermite_renormalized_diagonal (math/backend_numpy.py:423-432).
+ * A: 2M x 2M; B: 2M x nb (nb = 1 unbatched, else the reference's batch of
+ * B's); c: one value, or nb values if c_batched; G: cutoffs x nb (device,
+ * complex128, written in full).  reordered != 0 takes A and B in the
+ * [modes..., modes...] order and applies reorder_AB_bargmann
+ * (backend_numpy.py:367-375) on the fly; 0 takes them already in the
+ * [mode0, mode0, mode1, mode1, ...] order.  The workspace holds the four
+ * auxiliary sub-lattices, about (4M - 1) x prod(cutoffs) x nb complex128.
+ * The inner products are summed in index order, so results differ from the
+ * reference's BLAS dot at rounding level. */
+size_t mm_hermite_diagonal_workspace(const int64_t *cutoffs, int M, int64_t nb);
+int mm_hermite_diagonal(const double *A, const double *B, const double *c, int c_batched, const int64_t *cutoffs, int M,
+                        int64_t nb, int reordered, double *G, void *workspace, size_t ws_bytes, void *stream);
+
 /* ------------------------------------------------------ contraction (MFMA)
  *
  * C = A * B for complex128 row-major matrices (A: M x K with row stride lda,

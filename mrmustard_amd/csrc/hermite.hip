@@ -24,6 +24,7 @@
 #include "zgemm.hpp"
 #include "autoshape.hpp"
 #include "binomial.hpp"
+#include "diagonal.hpp"
 
 using namespace mm;
 
@@ -700,6 +701,77 @@ void launch_binomial_kernels(const Shape &S, const double2 *c, double max_l2, in
     hipLaunchKernelGGL((weight_mask_kernel<D>), dim3(unsigned(grid)), dim3(256), 0, s, G, g, wstop);
 }
 
+// ------------------------------------------------------------ diagonal path
+
+struct DiagGeom {
+    DiagArgs a;
+    int M = 0;
+    int64_t n2 = 0, n11 = 0;  // entries of arr2 and of arr1010 (each), per batch column
+    size_t t_bytes = 0, bytes = 0;
+};
+
+int diag_geom(const int64_t *cutoffs, int M, int64_t nb, DiagGeom *out) {
+    if (!cutoffs) return fail(MM_ERR_INVALID, "cutoffs pointer is NULL");
+    if (M < 1 || M > DG_MAXM) return fail(MM_ERR_INVALID, "diagonal needs 1 <= M <= %d modes, got %d", DG_MAXM, M);
+    if (nb < 1) return fail(MM_ERR_INVALID, "batch of %lld columns", (long long)nb);
+    DiagGeom g;
+    memset(&g.a, 0, sizeof(g.a));
+    g.M = M;
+    int64_t P = 1, S = 0;
+    for (int j = 0; j < M; ++j) {
+        if (cutoffs[j] < 1 || cutoffs[j] + 2 >= MM_MAX_CUTOFF)
+            return fail(MM_ERR_INVALID, "cutoffs[%d]=%lld outside [1, %d)", j, (long long)cutoffs[j], MM_MAX_CUTOFF - 2);
+        g.a.cut[j] = int(cutoffs[j]);
+        if (P > (int64_t(1) << 40) / (cutoffs[j] * nb)) return fail(MM_ERR_INVALID, "lattice too large");
+        P *= cutoffs[j];
+        S += cutoffs[j] - 1;
+    }
+    g.a.P = P;
+    g.a.nb = nb;
+    g.a.S = int(S);
+    int64_t len = 1;
+    for (int j = M - 1; j >= 0; --j) {
+        g.a.stride[j] = len;
+        len *= g.a.cut[j];
+        g.a.len[j] = len;
+    }
+    for (int d = 0; d < M; ++d) {
+        g.a.off2[d] = g.n2;
+        g.a.off11[d] = g.n11;
+        g.n2 += g.a.len[d];
+        g.n11 += int64_t(M - d - 1) * g.a.len[d];
+    }
+    g.t_bytes = align256(size_t(M + 2) * size_t(S + 1) * 8);  // T, then the prefix scratch
+    g.bytes = g.t_bytes + size_t(16) * size_t(nb) * size_t(2 * M * P + g.n2 + 2 * g.n11);
+    *out = g;
+    return MM_OK;
+}
+
+// T[0][w] for every weight, as the device table holds it (prefix sums per mode)
+std::vector<int64_t> diag_weight_counts(const DiagArgs &a, int M) {
+    const int W = a.S + 1;
+    std::vector<int64_t> row(W, 0), pre(W);
+    row[0] = 1;
+    for (int j = M - 1; j >= 0; --j) {
+        int64_t acc = 0;
+        for (int s = 0; s < W; ++s) pre[s] = (acc += row[s]);
+        for (int s = 0; s < W; ++s) row[s] = pre[s] - (s >= a.cut[j] ? pre[s - a.cut[j]] : 0);
+    }
+    return row;
+}
+
+template <int M, bool REORDER>
+int launch_diag_weights(const DiagArgs &a, const std::vector<int64_t> &counts, hipStream_t s) {
+    for (int w = 0; w <= a.S; ++w) {
+        const int64_t n = counts[w] * a.nb;
+        if (n == 0) continue;
+        const int64_t blocks = (n + 255) / 256;
+        if (blocks > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "weight %d has too many params for one launch", w);
+        hipLaunchKernelGGL((diag_weight_kernel<M, REORDER>), dim3(unsigned(blocks)), dim3(256), 0, s, a, w, counts[w]);
+    }
+    return MM_OK;
+}
+
 struct GlobalJob {
     Shape S;
     int64_t a_off, b_off, c_off, g_off;
@@ -1195,6 +1267,56 @@ int mm_autoshape(const double *A, const double *b, const double *c, int64_t B, i
     a.shape = shape;
     hipLaunchKernelGGL(autoshape_kernel, dim3(unsigned(B * M)), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), a);
     LAUNCH_CHECK("autoshape_kernel");
+    return MM_OK;
+}
+
+size_t mm_hermite_diagonal_workspace(const int64_t *cutoffs, int M, int64_t nb) {
+    DiagGeom g;
+    return diag_geom(cutoffs, M, nb, &g) ? 0 : g.bytes;
+}
+
+int mm_hermite_diagonal(const double *A, const double *B, const double *c, int c_batched, const int64_t *cutoffs, int M,
+                        int64_t nb, int reordered, double *G, void *workspace, size_t ws_bytes, void *stream) {
+    DiagGeom g;
+    int rc = diag_geom(cutoffs, M, nb, &g);
+    if (rc) return rc;
+    if (!A || !B || !c || !G) return fail(MM_ERR_INVALID, "NULL array pointer");
+    if (!workspace || ws_bytes < g.bytes)
+        return fail(MM_ERR_WORKSPACE, "diagonal workspace of %zu bytes needed, %zu given", g.bytes, ws_bytes);
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    DiagArgs &a = g.a;
+    a.A = reinterpret_cast<const double2 *>(A);
+    a.B = reinterpret_cast<const double2 *>(B);
+    a.c = reinterpret_cast<const double2 *>(c);
+    a.c_batched = c_batched ? 1 : 0;
+    a.sq = ds->sq;
+    a.rsq = ds->rsq;
+    char *ws = reinterpret_cast<char *>(workspace);
+    int64_t *T = reinterpret_cast<int64_t *>(ws);
+    int64_t *pre = T + int64_t(M + 1) * (a.S + 1);
+    a.T = T;
+    a.arr0 = reinterpret_cast<double2 *>(G);
+    a.arr1 = reinterpret_cast<double2 *>(ws + g.t_bytes);
+    a.arr2 = a.arr1 + 2 * M * a.P * nb;
+    a.a1010 = a.arr2 + g.n2 * nb;
+    a.a1001 = a.a1010 + g.n11 * nb;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(1024), 0, s, T, pre, a, M);
+    LAUNCH_CHECK("diag_counts_kernel");
+    const std::vector<int64_t> counts = diag_weight_counts(a, M);
+    switch (M * 2 + (reordered ? 1 : 0)) {
+#define CASE(MM)                                                       \
+    case 2 * MM: rc = launch_diag_weights<MM, false>(a, counts, s); break; \
+    case 2 * MM + 1: rc = launch_diag_weights<MM, true>(a, counts, s); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad M");
+    }
+    if (rc) return rc;
+    LAUNCH_CHECK("diag_weight_kernel");
     return MM_OK;
 }
 

@@ -21,6 +21,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from ..lattice import compact_fock as cf
 from ..lattice import strategies as st
 from ..settings import settings
 from .autograd import HermiteRenormalized, HermiteRenormalizedBatched, HermiteRenormalizedRecompute
@@ -113,6 +114,11 @@ class BackendHip:
             max_l2=max_l2 or settings.AUTOSHAPE_PROBABILITY,
             global_cutoff=global_cutoff or sum(shape) - len(shape) + 1,
         )[0]
+
+    def hermite_renormalized_diagonal(self, A, B, C, cutoffs: tuple[int], reorderedAB: bool):
+        """Photon-number diagonal (backend_numpy.py:423-432 -> inputValidation.py:61-79);
+        the [modes..., modes...] -> [mode0, mode0, ...] reorder happens in the kernel."""
+        return cf.diagonal_hip(A, B, C, cutoffs, reordered=reorderedAB)
 
     def hermite_renormalized_batched(self, A, b, c, shape: tuple[int], stable: bool = False, out=None):
         """Batched path (backend_numpy.py:394-403 -> batch.py:27)."""

@@ -158,6 +158,13 @@ class BackendManager:
         (default ``sum(shape) - len(shape) + 1``) (backend_manager.py:775-800)."""
         return self._apply("hermite_renormalized_binomial", (A, B, C, shape, max_l2, global_cutoff))
 
+    def hermite_renormalized_diagonal(self, A, b, c, cutoffs: tuple[int], reorderedAB: bool = True):
+        r"""The diagonal G[k0, k0, k1, k1, ...] of the lattice of (A, b, c): the
+        photon-number amplitudes of every mode, computed without the full
+        lattice; b may carry a batch of B's as its second axis
+        (backend_manager.py:729-757)."""
+        return self._apply("hermite_renormalized_diagonal", (A, b, c, cutoffs, reorderedAB))
+
     def _as_input(self, x):
         if hasattr(x, "ndim") and hasattr(x, "shape"):
             return x

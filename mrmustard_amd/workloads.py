@@ -190,3 +190,13 @@ def autoshape_states(n: int = 4096, m: int = 4, seed: int = 5):
         bs.append(b)
         cs.append(c)
     return np.stack(As), np.stack(bs), np.array(cs)
+
+
+def pnr_dm(m: int = 6, seed: int = 6):
+    """A normalised m-mode lossy Gaussian density matrix (A, b, c) in the
+    [bra modes..., ket modes...] order: its diagonal lattice entries are the
+    joint photon-number probabilities."""
+    rng = np.random.default_rng(seed)
+    A, b, c = gaussian_ket(m, rng, r_max=0.6, b_sigma=0.5)
+    dm = apply_loss(*ket_to_dm(A, b, c), rng.uniform(0.6, 1.0, m))
+    return normalize_dm(*dm)

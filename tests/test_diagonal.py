@@ -1,0 +1,224 @@
+"""The diagonal strategy (compactFock): the oracle pinned on the CPU, the HIP
+path against it on the GPU.
+
+Mirrors test_compactFock.py:19-44: the diagonal amplitudes of
+(conj(-A), conj(b), conj(c)) of a random density matrix equal the entries
+G[k, k] of the full 2M-index lattice of the same triple.  Closed forms pin the
+rest: a coherent density matrix has Poisson photon numbers, a thermal one
+geometric, a product state the product of its modes'.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from mrmustard_amd import workloads as W
+from tests.conftest import gpu_available
+
+TOL = 1e-12  # normwise, max|got - ref| / max|ref|: the inner products' summation order differs
+
+
+def random_dm(M, seed, loss=True):
+    rng = np.random.default_rng(seed)
+    A, b, c = W.gaussian_ket(M, rng, r_max=0.6, b_sigma=0.5)
+    dm = W.ket_to_dm(A, b, c)
+    if loss:
+        dm = W.apply_loss(*dm, rng.uniform(0.6, 1.0, M))
+    return W.normalize_dm(*dm)
+
+
+def reference_triple(M, seed):
+    """(conj(-A), conj(b), conj(c)) as test_compactFock.py:25-37 feeds both paths."""
+    A, b, c = random_dm(M, seed)
+    return np.conj(-np.asarray(A)), np.conj(np.asarray(b)), np.conj(complex(c))
+
+
+def full_diagonal(oracle_mod, A, b, c, cutoffs):
+    full = oracle_mod.vanilla(tuple(cutoffs) * 2, A, b, c)
+    out = np.zeros(cutoffs, complex)
+    for k in np.ndindex(*cutoffs):
+        out[k] = full[k + k]
+    return out
+
+
+def coherent_dm(alphas):
+    """Product of coherent states: A = 0, b = (conj(alpha), alpha), c = exp(-sum |alpha|^2)."""
+    al = np.asarray(alphas, complex)
+    M = len(al)
+    return np.zeros((2 * M, 2 * M), complex), np.concatenate([np.conj(al), al]), np.exp(-np.sum(np.abs(al) ** 2))
+
+
+def thermal_dm(nbars):
+    """Product of thermal states: exp(sum_j t_j conj(z_j) z_j) / prod(1 + n_j), t = n / (1 + n)."""
+    n = np.asarray(nbars, float)
+    M = len(n)
+    A = np.zeros((2 * M, 2 * M), complex)
+    for j in range(M):
+        A[j, M + j] = A[M + j, j] = n[j] / (1 + n[j])
+    return A, np.zeros(2 * M, complex), 1.0 / np.prod(1 + n)
+
+
+def poisson(lam, K):
+    return np.array([math.exp(-lam) * lam**k / math.factorial(k) for k in range(K)])
+
+
+def relerr(got, ref):
+    return np.max(np.abs(np.asarray(got) - ref)) / max(np.max(np.abs(ref)), 1e-300)
+
+
+# ----------------------------------------------------------------- oracle
+
+
+@pytest.mark.parametrize("M,cutoffs,seed", [(1, (9,), 1), (2, (5, 5), 2), (3, (5, 5, 5), 3), (3, (3, 6, 4), 4),
+                                            (2, (1, 6), 5), (2, (6, 1), 6), (3, (4, 1, 3), 7)])
+def test_oracle_vs_full_lattice(oracle_mod, M, cutoffs, seed):
+    """test_compactFock.py:19-44 with the pinned vanilla oracle as the full lattice."""
+    from oracle import diagonal as OD
+
+    A, b, c = reference_triple(M, seed)
+    ref = full_diagonal(oracle_mod, A, b, c, cutoffs)
+    Ar, br = OD.reorder_ab(A, b)
+    assert relerr(OD.diagonal(Ar, br, c, cutoffs), ref) < 1e-13
+
+
+def test_oracle_closed_forms():
+    from oracle import diagonal as OD
+
+    alphas = [0.7, 1.1 - 0.4j]
+    Ar, br = OD.reorder_ab(*coherent_dm(alphas)[:2])
+    got = OD.diagonal(Ar, br, coherent_dm(alphas)[2], (12, 10))
+    want = np.outer(poisson(abs(alphas[0]) ** 2, 12), poisson(abs(alphas[1]) ** 2, 10))
+    assert np.max(np.abs(got - want)) < 1e-15
+    A, b, c = thermal_dm([0.5, 2.0])
+    Ar, br = OD.reorder_ab(A, b)
+    got = OD.diagonal(Ar, br, c, (15, 15))
+    t = np.array([0.5 / 1.5, 2.0 / 3.0])
+    want = c * np.outer(t[0] ** np.arange(15), t[1] ** np.arange(15))
+    assert np.max(np.abs(got - want)) < 1e-15
+
+
+# ----------------------------------------------------------------- GPU
+
+GPU_CASES = [
+    (1, (40,), 11),
+    (2, (5, 5), 12),
+    (3, (5, 5, 5), 13),
+    (3, (3, 7, 4), 14),
+    (2, (1, 6), 15),
+    (3, (4, 1, 3), 16),
+    (4, (6, 5, 4, 6), 17),
+    (5, (4, 4, 4, 4, 4), 18),
+    (6, (3, 3, 3, 3, 3, 3), 19),
+    (8, (2, 3, 2, 2, 3, 2, 2, 2), 20),
+    (10, (2,) * 10, 21),
+]
+
+
+def gpu_oracle(M, cutoffs, A, b, c):
+    from oracle import diagonal as OD
+
+    Ar, br = OD.reorder_ab(A, b)
+    return OD.diagonal(Ar, br, c, cutoffs)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("M,cutoffs,seed", GPU_CASES, ids=lambda x: str(x))
+def test_diagonal_hip_vs_oracle(M, cutoffs, seed):
+    from oracle import diagonal as OD
+
+    from mrmustard_amd.lattice.compact_fock import diagonal_hip
+
+    A, b, c = reference_triple(M, seed)
+    print(f"diagonal M={M} cutoffs={cutoffs}", flush=True)
+    ref = gpu_oracle(M, cutoffs, A, b, c)
+    got = diagonal_hip(A, b, c, cutoffs, reordered=True).cpu().numpy()
+    assert got.shape == tuple(cutoffs)
+    assert relerr(got, ref) < TOL
+    Ar, br = OD.reorder_ab(A, b)
+    got2 = diagonal_hip(Ar, br, c, cutoffs, reordered=False).cpu().numpy()
+    assert relerr(got2, ref) < TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("c_batched", [False, True])
+def test_diagonal_hip_batched_b(c_batched):
+    """B of shape (2M, nb): each column its own lattice (diagonal_amps.py:226-236)."""
+    from mrmustard_amd.lattice.compact_fock import diagonal_hip
+
+    M, cutoffs, nb = 3, (4, 5, 3), 7
+    A, b, c = reference_triple(M, 31)
+    rng = np.random.default_rng(32)
+    Bm = b[:, None] + 0.3 * (rng.standard_normal((2 * M, nb)) + 1j * rng.standard_normal((2 * M, nb)))
+    cs = c * (1 + 0.1 * rng.standard_normal(nb)) if c_batched else c
+    got = diagonal_hip(A, Bm, cs, cutoffs, reordered=True).cpu().numpy()
+    assert got.shape == cutoffs + (nb,)
+    for k in range(nb):
+        ref = gpu_oracle(M, cutoffs, A, Bm[:, k], cs[k] if c_batched else c)
+        assert relerr(got[..., k], ref) < TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+def test_diagonal_hip_closed_forms():
+    """Poisson and geometric photon numbers at cutoffs the oracle would take long on."""
+    from mrmustard_amd.lattice.compact_fock import diagonal_hip
+
+    alphas = [2.0, 1.5 + 1.0j, 0.5j]
+    A, b, c = coherent_dm(alphas)
+    got = diagonal_hip(A, b, c, (40, 30, 8), reordered=True).cpu().numpy()
+    want = np.einsum("i,j,k->ijk", *[poisson(abs(a) ** 2, K) for a, K in zip(alphas, (40, 30, 8))])
+    assert np.max(np.abs(got - want)) < 1e-14
+    A, b, c = thermal_dm([1.0, 3.0])
+    got = diagonal_hip(A, b, c, (100, 80), reordered=True).cpu().numpy()
+    t = np.array([0.5, 0.75])
+    want = c * np.outer(t[0] ** np.arange(100), t[1] ** np.arange(80))
+    assert np.max(np.abs(got - want)) < 1e-14
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("M,K", [(2, 30), (4, 10)])
+def test_diagonal_hip_vs_full_lattice_hip(M, K):
+    """The reference's own check at sizes beyond the oracle: the diagonal of the
+    full 2M-index lattice from the (bit-exact) vanilla kernel."""
+    from mrmustard_amd import math
+
+    A, b, c = reference_triple(M, 40 + M)
+    cut = (K,) * M
+    full = math.hermite_renormalized(A, b, c, shape=cut * 2)
+    import torch
+
+    grids = torch.meshgrid(*([torch.arange(K, device=full.device)] * M), indexing="ij")
+    ref = full[tuple(grids) * 2].cpu().numpy()
+    got = math.hermite_renormalized_diagonal(A, b, c, cut)
+    assert relerr(got.cpu().numpy(), ref) < TOL
+    # a valid density matrix: real, non-negative photon-number probabilities summing to at most 1
+    A0, b0, c0 = random_dm(M, 40 + M)
+    p = math.hermite_renormalized_diagonal(A0, b0, c0, cut).cpu().numpy()
+    assert np.max(np.abs(p.imag)) < 1e-12 and p.real.min() > -1e-12 and p.real.sum() < 1 + 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+def test_diagonal_hip_errors():
+    from mrmustard_amd.lattice.compact_fock import diagonal_hip
+
+    A, b, c = reference_triple(2, 1)
+    bad = A.copy()
+    bad[0, 1] += 1.0
+    with pytest.raises(ValueError, match="symmetric"):
+        diagonal_hip(bad, b, c, (3, 3))
+    with pytest.raises(ValueError, match="incompatible"):
+        diagonal_hip(A, b[:3], c, (3, 3))
+    with pytest.raises(ValueError, match="incompatible"):
+        diagonal_hip(A, b, c, (3, 3, 3))
+    with pytest.raises(ValueError):
+        diagonal_hip(A, b, c, (0, 3))
+    A11, b11, c11 = coherent_dm([0.1] * 11)
+    with pytest.raises(ValueError):
+        diagonal_hip(A11, b11, c11, (2,) * 11)
