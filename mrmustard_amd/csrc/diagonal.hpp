@@ -15,16 +15,28 @@
 //
 // Every value written at weight w is read by exactly one params of weight
 // w + 1 and nothing of weight w reads another params' weight-w values, so one
-// launch per weight runs all params of that weight in parallel, a thread per
-// (params, batch column).  A thread finds its params from its rank within the
-// weight by unranking against a count table T[j][s] = #{(p_j..p_{M-1}) : p_k <
-// cutoff_k, sum = s}, in np.ndindex order (helperFunctions.py:31-46).
+// launch per weight runs all params of that weight in parallel.  Within a
+// launch a params is a group of 2M lanes: lane i holds the i-th input of a
+// pivot, the group exchanges them through LDS, and lane i forms output i (the
+// dot of A's row i with the inputs): every output of both pivot kinds is row
+// i of A for lane i.
 //
-// Layout: arrays are [slice][flat p][batch column], batch innermost.  Slices
-// of arr2 / arr1010 / arr1001 for d are only ever touched at params with
-// p[:d] == 0, whose flat index is below prod(cutoffs[d:]), so slice d is
-// stored with that length: the workspace is about (2M + 2(M - 1) + 1)
-// lattices instead of the reference's (2M^2 + M).
+// Params are numbered by their position in a list sorted by weight (np.ndindex
+// order within a weight, helperFunctions.py:31-46), so the params of a launch
+// are contiguous.  The producer of a value writes it straight into its one
+// consumer's inbox, at the consumer's position: lane i of a pivot at p sends
+// its output to slot i ^ 1 of p + e_m (m = i / 2), for the diagonal pivot's
+// arr1 values (the consumer's diagonal inbox) and for the d = 0 off-diagonal
+// pivot's arr0 / arr2 / arr1010 / arr1001 values (the consumer's off-diagonal
+// inbox; slot 0 holds arr0[p], read as the consumer's G[p ⊗ p]).  A consumer
+// then reads all its inputs at its own position: one coalesced round trip.
+// Values of the off-diagonal pivots d >= 1, met only by params with p[0] == 0,
+// stay in the reference's sub-lattices (indexed by flat p, of length
+// prod(cutoffs[d:]) since p[:d] == 0) and are gathered at p - e_m.
+//
+// Positions come from count tables: T[j][s] = #{(p_j..p_{M-1}) : p_k <
+// cutoff_k, sum = s} and their row prefix sums Q give the position of p, or of
+// p + e_m, in O(M) (list_position).
 #pragma once
 
 #include "common.hpp"
@@ -32,173 +44,275 @@
 namespace mm {
 
 constexpr int DG_MAXM = 10;
+constexpr int DG_THREADS = 256;
 
 struct DiagArgs {
     const double2 *A, *B, *c;  // A: 2M x 2M; B: 2M x nb; c: 1 or nb values
     int c_batched;
-    double2 *arr0, *arr1, *arr2, *a1010, *a1001;
-    const int64_t *T;  // (M + 1) x (S + 1) counts
+    double2 *arr0;              // output, by flat p
+    double2 *inD, *inO;         // 2M x P inboxes by position: diagonal and d = 0 off-diagonal inputs
+    double2 *arr2, *a1010, *a1001;  // d >= 1 sub-lattices, by flat p
+    int64_t *T, *Q;     // (M + 1) x (S + 1) counts and their row prefix sums (global copy, large S only)
+    int32_t *list;      // P < 2^31 flat indices, sorted by weight, ndindex order within a weight
+    int32_t *fwd;       // M x P: list position of p + e_m for the params at each position (-1: outside)
     const double *sq, *rsq;
     int64_t P, nb;
     int S;
     int cut[DG_MAXM];
     int64_t stride[DG_MAXM];
     int64_t len[DG_MAXM];   // prod(cut[d:])
-    int64_t off2[DG_MAXM];  // start of arr2's slice d (entries)
-    int64_t off11[DG_MAXM]; // start of arr1010/arr1001's block d (M - d - 1 slices of len[d])
+    int64_t off2[DG_MAXM];  // start of arr2's slice d >= 1 (entries)
+    int64_t off11[DG_MAXM]; // start of arr1010/arr1001's block d >= 1 (M - d - 1 slices of len[d])
 };
 
-// Count table: T[M][s] = (s == 0); T[j][s] = sum_{v < cut_j, v <= s} T[j+1][s-v]
-// = pre[s] - pre[s - cut_j] with pre the inclusive prefix sum of row j+1.
-// One block; the prefix sum is chunked per thread (pre: W int64 of scratch).
-__global__ __launch_bounds__(1024) void diag_counts_kernel(int64_t *T, int64_t *pre, DiagArgs a, int M) {
-    __shared__ int64_t part[1024];
-    const int W = a.S + 1, nt = blockDim.x, tid = threadIdx.x;
-    const int chunk = (W + nt - 1) / nt;
-    const int lo = min(W, tid * chunk), hi = min(W, lo + chunk);
-    for (int s = tid; s < W; s += nt) T[int64_t(M) * W + s] = (s == 0);
-    for (int j = M - 1; j >= 0; --j) {
-        __syncthreads();
-        const int64_t *row = T + int64_t(j + 1) * W;
-        int64_t acc = 0;
-        for (int s = lo; s < hi; ++s) acc += row[s];
-        part[tid] = acc;
-        __syncthreads();
-        if (tid == 0) {
-            int64_t run = 0;
-            for (int k = 0; k < nt; ++k) {
-                const int64_t v = part[k];
-                part[k] = run;
-                run += v;
-            }
+__device__ __forceinline__ int64_t shfl_up_i64(int64_t v, int delta) {
+    int lo = int(uint64_t(v) & 0xffffffffu), hi = int(uint64_t(v) >> 32);
+    lo = __shfl_up(lo, delta, MM_WAVE);
+    hi = __shfl_up(hi, delta, MM_WAVE);
+    return int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
+// T[M][s] = (s == 0); Q[j] = prefix(T[j]); T[j-1][s] = Q[j][s] - Q[j][s - cut_{j-1}],
+// down to Q[0] (the start of each weight in the list).  Run by the 64 lanes of
+// one wave (a chunk of each row per lane, a shuffle scan of the chunk sums), so
+// no block barrier is needed; T and Q may point to LDS or to global memory.
+__device__ void diag_build_tables(const DiagArgs &a, int M, int64_t *T, int64_t *Q) {
+    const int W = a.S + 1, lane = threadIdx.x & (MM_WAVE - 1);
+    const int chunk = (W + MM_WAVE - 1) / MM_WAVE;
+    const int lo = min(W, lane * chunk), hi = min(W, lo + chunk);
+    for (int s = lane; s < W; s += MM_WAVE) T[int64_t(M) * W + s] = (s == 0);
+    wave_sync();
+    for (int j = M; j >= 0; --j) {
+        const int64_t *row = T + int64_t(j) * W;
+        int64_t *out = Q + int64_t(j) * W;
+        int64_t own = 0;
+        for (int s = lo; s < hi; ++s) own += row[s];
+        int64_t incl = own;
+#pragma unroll
+        for (int d = 1; d < MM_WAVE; d <<= 1) {
+            const int64_t v = shfl_up_i64(incl, d);
+            if (lane >= d) incl += v;
         }
-        __syncthreads();
-        acc = part[tid];
+        int64_t acc = incl - own;
         for (int s = lo; s < hi; ++s) {
             acc += row[s];
-            pre[s] = acc;
+            out[s] = acc;
         }
+        wave_sync();
+        if (j == 0) break;
+        const int cj = a.cut[j - 1];
+        for (int s = lane; s < W; s += MM_WAVE) T[int64_t(j - 1) * W + s] = out[s] - (s >= cj ? out[s - cj] : 0);
+        wave_sync();
+    }
+}
+
+// Global tables, for count tables too large for LDS (one wave).
+__global__ __launch_bounds__(64) void diag_counts_kernel(DiagArgs a, int M) { diag_build_tables(a, M, a.T, a.Q); }
+
+// List position of p (m < 0) or of p + e_m from the prefix tables:
+//   start(w') + sum_{j < M-1} C_j(x_j, t_j),  C_j(x, t) = Q[j+1][t] - Q[j+1][t - x],
+// with w' = |p| or |p| + 1 and (x_j, t_j) the digits and suffix sums of that
+// params: for p + e_m, t_j = s_j + 1 for j <= m and x_m = p_m + 1.
+template <int M>
+__device__ __forceinline__ int64_t list_position(const int64_t *Q, int W, const int *p, int w, int m) {
+    const int wp = m < 0 ? w : w + 1;
+    int64_t g = wp > 0 ? Q[wp - 1] : 0;
+    int s = w;
+#pragma unroll
+    for (int j = 0; j < M - 1; ++j) {
+        const int64_t *Qr = Q + int64_t(j + 1) * W;
+        const int hi = j <= m ? s + 1 : s;
+        const int lo = j < m ? s + 1 - p[j] : s - p[j];
+        g += Qr[hi] - Qr[lo];
+        s -= p[j];
+    }
+    return g;
+}
+
+// For every p of the box at list position g: list[g] = flat(p) and
+// fwd[m][g] = list position of p + e_m (-1 where p_m + 1 == cutoff_m).  With lds_tables
+// each block first builds T and Q in its dynamic LDS; otherwise it reads the
+// global Q that diag_counts_kernel built.
+template <int M>
+__global__ __launch_bounds__(256) void diag_rank_kernel(DiagArgs a, int lds_tables) {
+    extern __shared__ int64_t tab[];
+    const int W = a.S + 1;
+    const int64_t *Q = a.Q;
+    if (lds_tables) {
+        if (threadIdx.x < MM_WAVE) diag_build_tables(a, M, tab, tab + int64_t(M + 1) * W);
         __syncthreads();
-        const int cj = a.cut[j];
-        for (int s = tid; s < W; s += nt) T[int64_t(j) * W + s] = pre[s] - (s >= cj ? pre[s - cj] : 0);
+        Q = tab + int64_t(M + 1) * W;
+    }
+    const uint32_t P = uint32_t(a.P);
+    const uint32_t inc = gridDim.x * blockDim.x;
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < P; f += inc) {
+        int p[M];
+        uint32_t r = f;
+        int w = 0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const uint32_t sj = uint32_t(a.stride[j]);
+            p[j] = int(r / sj);
+            r -= uint32_t(p[j]) * sj;
+            w += p[j];
+        }
+        const int64_t g = list_position<M>(Q, W, p, w, -1);
+        a.list[g] = int32_t(f);
+#pragma unroll
+        for (int m = 0; m < M; ++m) a.fwd[m * a.P + g] = p[m] + 1 < a.cut[m] ? int32_t(list_position<M>(Q, W, p, w, m)) : -1;
     }
 }
 
 // Compact index k -> index into the caller's A/b: identity, or the reference's
 // reorder_AB_bargmann (backend_numpy.py:367-375) from [modes..., modes...].
 template <int M, bool REORDER>
-__device__ __forceinline__ constexpr int src_index(int k) {
+__device__ __forceinline__ int src_index(int k) {
     return REORDER ? (k & 1) * M + (k >> 1) : k;
 }
 
+// One launch per weight w: PB params per block, lane (slot, i) = (t % PB, t / PB)
+// so that a wave's loads of one inbox slot cover consecutive positions.  All
+// inputs of all pivots are loaded at once and parked in LDS, one slot per
+// pivot: 0 for the diagonal pivot, 1 + d for the off-diagonal pivot d.  A sits
+// in LDS too (lane i reads row i).  Two barriers: after the inputs land, and
+// after the diagonal pivot's own outputs arr1[2d, p] (inputs of the
+// off-diagonal pivots) are in LDS.
 template <int M, bool REORDER>
-__device__ __forceinline__ double2 a_at(const double2 *A, int i, int j) {
-    return A[src_index<M, REORDER>(i) * (2 * M) + src_index<M, REORDER>(j)];
-}
+__global__ __launch_bounds__(DG_THREADS) void diag_group_kernel(DiagArgs a, int w, int64_t count, int64_t start) {
+    constexpr int G = 2 * M;            // lanes per params
+    constexpr int PB = DG_THREADS / G;  // params per block
+    __shared__ double2 sgin[M + 1][G][PB + 1];
+    __shared__ double2 sev[M][PB];
+    __shared__ double2 sA[G][G];  // A in the compact order
+    const int t = threadIdx.x;
+    for (int e = t; e < G * G; e += DG_THREADS)
+        sA[e / G][e % G] = a.A[src_index<M, REORDER>(e / G) * G + src_index<M, REORDER>(e % G)];
+    const int slot = t % PB, i = t / PB, m = i >> 1;
+    const bool lane_ok = i < G;
+    const int64_t nb = a.nb, P = a.P;
+    const int64_t grp = int64_t(blockIdx.x) * PB + slot;
+    const bool act = lane_ok && grp < count * nb;
 
-// (g B_i + A_i . gin) / sqrt(k), the dot over j >= j0 (gin vanishes below)
-template <int M, bool REORDER>
-__device__ __forceinline__ double2 pivot_out(const DiagArgs &a, int i, int j0, double2 g, const double2 *gin, int64_t col, int k) {
-    double2 dot = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int j = 0; j < 2 * M; ++j)
-        if (j >= j0) dot = cadd(dot, cmul(a_at<M, REORDER>(a.A, i, j), gin[j]));
-    const double2 t = cadd(cmul(g, a.B[int64_t(src_index<M, REORDER>(i)) * a.nb + col]), dot);
-    return cdiv_by(t, a.sq[k], a.rsq[k]);
-}
-
-template <int M, bool REORDER>
-__global__ __launch_bounds__(256) void diag_weight_kernel(DiagArgs a, int w, int64_t count) {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t nb = a.nb;
-    if (t >= count * nb) return;
-    int64_t r = t / nb;
-    const int64_t col = t - r * nb;
-
-    // unrank: params of weight w in np.ndindex order
-    const int W = a.S + 1;
+    int64_t f = 0, col = 0, gs = 0, fw = -1;
+    double2 g0 = make_double2(0.0, 0.0), xD = make_double2(0.0, 0.0), xO = make_double2(0.0, 0.0);
     int p[M];
-    int s = w;
 #pragma unroll
-    for (int j = 0; j < M - 1; ++j) {
-        const int top = s < a.cut[j] - 1 ? s : a.cut[j] - 1;
-        int v = 0;
-        for (; v < top; ++v) {
-            const int64_t n = a.T[int64_t(j + 1) * W + (s - v)];
-            if (r < n) break;
-            r -= n;
+    for (int j = 0; j < M; ++j) p[j] = 0;
+    if (act) {
+        const int64_t q = nb == 1 ? grp : grp / nb;
+        col = grp - q * nb;
+        gs = start + q;
+        f = a.list[gs];
+        fw = a.fwd[m * P + gs];  // position of p + e_m, where this lane's outputs go
+        if (w == 0) {
+            g0 = a.c[a.c_batched ? col : 0];
+            if (i == 0) a.arr0[col] = g0;
+        } else {
+            g0 = a.inO[gs * nb + col];  // slot 0: arr0[p]
+            xD = a.inD[(i * P + gs) * nb + col];
+            xO = a.inO[(i * P + gs) * nb + col];
         }
-        p[j] = v;
-        s -= v;
-    }
-    p[M - 1] = s;
-    int64_t f = 0;
+        uint32_t r = uint32_t(f);
 #pragma unroll
-    for (int j = 0; j < M; ++j) f += p[j] * a.stride[j];
-    const int64_t P = a.P;
-
-    double2 g0;
-    if (w == 0) {
-        g0 = a.c[a.c_batched ? col : 0];
-        a.arr0[col] = g0;
-    } else {
-        g0 = a.arr0[f * nb + col];
-    }
-
-    // diagonal pivot (diagonal_amps.py:108-150)
-    double2 ev[M];  // arr1[2d, p], read back by the off-diagonal pivots
-#pragma unroll
-    for (int m = 0; m < M; ++m) ev[m] = make_double2(0.0, 0.0);
-    if (a.cut[0] == 1 || p[0] < a.cut[0] - 1) {
-        double2 gin[2 * M];
-#pragma unroll
-        for (int i = 0; i < 2 * M; ++i) {
-            const int m = i >> 1;
-            gin[i] = make_double2(0.0, 0.0);
-            if (p[m] > 0) {
-                const double2 v = a.arr1[((i ^ 1) * P + f - a.stride[m]) * nb + col];
-                gin[i] = cscale_promoted(v, a.sq[p[m]]);
-            }
+        for (int j = 0; j < M; ++j) {
+            const uint32_t sj = uint32_t(a.stride[j]);
+            p[j] = int(r / sj);
+            r -= uint32_t(p[j]) * sj;
         }
+    }
+    // this lane's mode: p[m], stride[m], cut[m] without dynamic register indexing
+    int pm = 0, cm = 1;
+    int64_t sm = 0;
 #pragma unroll
-        for (int i = 0; i < 2 * M; ++i) {
-            const int m = i >> 1;
-            if (p[m] + 1 < a.cut[m] && (i != 1 || p[0] + 2 < a.cut[0])) {
-                const double2 v = pivot_out<M, REORDER>(a, i, 0, g0, gin, col, p[m] + 1);
-                a.arr1[(i * P + f) * nb + col] = v;
-                if (!(i & 1)) ev[m] = v;
-            }
+    for (int j = 0; j < M; ++j)
+        if (j == m) {
+            pm = p[j];
+            cm = a.cut[j];
+            sm = a.stride[j];
+        }
+    // off-diagonal pivots this params runs: d with p[:d] == 0 and p[d] < cut[d] - 1
+    int runmask = 0;
+    if (act) {
+        bool zp = true;
+#pragma unroll
+        for (int d = 0; d < M; ++d) {
+            if (zp && p[d] < a.cut[d] - 1) runmask |= 1 << d;
+            zp = zp && p[d] == 0;
         }
     }
 
-    // off-diagonal pivots for every d with p[:d] == 0 (diagonal_amps.py:19-105)
+    // ---- inputs, scaled by sqrt of the pivot's entries (K_l)
+    const bool diag = act && (a.cut[0] == 1 || p[0] < a.cut[0] - 1);
+    if (lane_ok) sgin[0][i][slot] = diag && pm > 0 ? cscale_promoted(xD, a.sq[pm]) : make_double2(0.0, 0.0);
 #pragma unroll
     for (int d = 0; d < M; ++d) {
-        if (d > 0 && p[d - 1] != 0) break;
-        if (p[d] >= a.cut[d] - 1) continue;
-        const double2 g = ev[d];
-        double2 gin[2 * M];
-#pragma unroll
-        for (int j = 0; j < 2 * M; ++j) gin[j] = make_double2(0.0, 0.0);
-        gin[2 * d] = cscale_promoted(g0, a.sq[p[d] + 1]);
-        if (p[d] > 0) gin[2 * d + 1] = cscale_promoted(a.arr2[(a.off2[d] + f - a.stride[d]) * nb + col], a.sq[p[d]]);
-#pragma unroll
-        for (int i = d + 1; i < M; ++i) {
-            if (p[i] > 0) {
-                const int64_t e = (a.off11[d] + (i - d - 1) * a.len[d] + f - a.stride[i]) * nb + col;
-                gin[2 * i] = cscale_promoted(a.a1001[e], a.sq[p[i]]);
-                gin[2 * i + 1] = cscale_promoted(a.a1010[e], a.sq[p[i]]);
+        if (!lane_ok || i < 2 * d) continue;
+        double2 v = make_double2(0.0, 0.0);
+        if (runmask >> d & 1) {
+            if (i == 2 * d) {
+                v = cscale_promoted(g0, a.sq[p[d] + 1]);
+            } else if (pm > 0) {
+                double2 x = xO;  // d == 0: the inbox
+                if (d > 0) {
+                    if (i == 2 * d + 1) {
+                        x = a.arr2[(a.off2[d] + f - sm) * nb + col];
+                    } else {
+                        const int64_t en = (a.off11[d] + (m - d - 1) * a.len[d] + f - sm) * nb + col;
+                        x = (i & 1) ? a.a1010[en] : a.a1001[en];
+                    }
+                }
+                v = cscale_promoted(x, a.sq[pm]);
             }
         }
-        a.arr0[(f + a.stride[d]) * nb + col] = pivot_out<M, REORDER>(a, 2 * d + 1, 2 * d, g, gin, col, p[d] + 1);
-        if (p[d] + 2 < a.cut[d])
-            a.arr2[(a.off2[d] + f) * nb + col] = pivot_out<M, REORDER>(a, 2 * d, 2 * d, g, gin, col, p[d] + 2);
+        sgin[1 + d][i][slot] = v;
+    }
+    double2 bi = make_double2(0.0, 0.0);
+    if (act) bi = a.B[int64_t(src_index<M, REORDER>(i)) * nb + col];
+    __syncthreads();
+
+    // ---- diagonal pivot (diagonal_amps.py:108-150): arr1[i, p] -> slot i ^ 1 of p + e_m
+    if (diag && pm + 1 < cm && (i != 1 || p[0] + 2 < a.cut[0])) {
+        double2 dot = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int i = d + 1; i < M; ++i) {
-            if (p[i] + 1 < a.cut[i]) {
-                const int64_t e = (a.off11[d] + (i - d - 1) * a.len[d] + f) * nb + col;
-                a.a1010[e] = pivot_out<M, REORDER>(a, 2 * i, 2 * d, g, gin, col, p[i] + 1);
-                a.a1001[e] = pivot_out<M, REORDER>(a, 2 * i + 1, 2 * d, g, gin, col, p[i] + 1);
+        for (int j = 0; j < G; ++j) dot = cadd(dot, cmul(sA[i][j], sgin[0][j][slot]));
+        const double2 out = cdiv_by(cadd(cmul(g0, bi), dot), a.sq[pm + 1], a.rsq[pm + 1]);
+        a.inD[((i ^ 1) * P + fw) * nb + col] = out;
+        if (!(i & 1)) sev[m][slot] = out;
+    }
+    __syncthreads();
+
+    // ---- off-diagonal pivots (diagonal_amps.py:19-105)
+#pragma unroll
+    for (int d = 0; d < M; ++d) {
+        if (!(runmask >> d & 1) || i < 2 * d) continue;
+        const int pd = p[d];
+        double2 dot = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 2 * d; j < G; ++j) dot = cadd(dot, cmul(sA[i][j], sgin[1 + d][j][slot]));
+        const double2 tt = cadd(cmul(sev[d][slot], bi), dot);
+        if (i == 2 * d + 1) {
+            // arr0[p + e_d]: the output, and slot 0 of that params' inbox
+            const double2 o = cdiv_by(tt, a.sq[pd + 1], a.rsq[pd + 1]);
+            a.arr0[(f + a.stride[d]) * nb + col] = o;
+            a.inO[fw * nb + col] = o;
+        } else if (i == 2 * d) {
+            if (pd + 2 < a.cut[d]) {
+                const double2 o = cdiv_by(tt, a.sq[pd + 2], a.rsq[pd + 2]);
+                if (d == 0)
+                    a.inO[(P + fw) * nb + col] = o;  // slot 1 of p + e_0
+                else
+                    a.arr2[(a.off2[d] + f) * nb + col] = o;
+            }
+        } else if (pm + 1 < cm) {
+            const double2 o = cdiv_by(tt, a.sq[pm + 1], a.rsq[pm + 1]);
+            if (d == 0) {
+                a.inO[((i ^ 1) * P + fw) * nb + col] = o;
+            } else {
+                const int64_t e = (a.off11[d] + (m - d - 1) * a.len[d] + f) * nb + col;
+                if (i & 1)
+                    a.a1001[e] = o;
+                else
+                    a.a1010[e] = o;
             }
         }
     }

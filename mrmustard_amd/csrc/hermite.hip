@@ -706,7 +706,7 @@ void launch_binomial_kernels(const Shape &S, const double2 *c, double max_l2, in
 struct DiagGeom {
     DiagArgs a;
     int M = 0;
-    int64_t n2 = 0, n11 = 0;  // entries of arr2 and of arr1010 (each), per batch column
+    int64_t n2 = 0, n11 = 0;  // entries of the d >= 1 arr2 and arr1010 (each), per batch column
     size_t t_bytes = 0, bytes = 0;
 };
 
@@ -726,6 +726,7 @@ int diag_geom(const int64_t *cutoffs, int M, int64_t nb, DiagGeom *out) {
         P *= cutoffs[j];
         S += cutoffs[j] - 1;
     }
+    if (P > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "diagonal: prod(cutoffs) = %lld is not below 2^31", (long long)P);
     g.a.P = P;
     g.a.nb = nb;
     g.a.S = int(S);
@@ -735,14 +736,15 @@ int diag_geom(const int64_t *cutoffs, int M, int64_t nb, DiagGeom *out) {
         len *= g.a.cut[j];
         g.a.len[j] = len;
     }
-    for (int d = 0; d < M; ++d) {
+    for (int d = 1; d < M; ++d) {  // sub-lattices of the off-diagonal pivots d >= 1
         g.a.off2[d] = g.n2;
         g.a.off11[d] = g.n11;
         g.n2 += g.a.len[d];
         g.n11 += int64_t(M - d - 1) * g.a.len[d];
     }
-    g.t_bytes = align256(size_t(M + 2) * size_t(S + 1) * 8);  // T, then the prefix scratch
-    g.bytes = g.t_bytes + size_t(16) * size_t(nb) * size_t(2 * M * P + g.n2 + 2 * g.n11);
+    g.t_bytes = align256(size_t(2 * (M + 1)) * size_t(S + 1) * 8) + align256(size_t(P) * 4) +
+                align256(size_t(M) * size_t(P) * 4);  // T and Q, list, nbr
+    g.bytes = g.t_bytes + size_t(16) * size_t(nb) * size_t(4 * M * P + g.n2 + 2 * g.n11);  // two inboxes + sub-lattices
     *out = g;
     return MM_OK;
 }
@@ -760,14 +762,27 @@ std::vector<int64_t> diag_weight_counts(const DiagArgs &a, int M) {
     return row;
 }
 
+// Bytes of LDS for the count and prefix tables in each rank block, or 0 when
+// they are too large and come from diag_counts_kernel in global memory.
+size_t diag_table_lds(const DiagArgs &a, int M) {
+    const size_t b = size_t(2 * (M + 1)) * size_t(a.S + 1) * 8;
+    return b <= 48 * 1024 ? b : 0;
+}
+
 template <int M, bool REORDER>
-int launch_diag_weights(const DiagArgs &a, const std::vector<int64_t> &counts, hipStream_t s) {
+int launch_diag_weights(const DiagArgs &a, const std::vector<int64_t> &counts, int num_cu, hipStream_t s) {
+    constexpr int PB = DG_THREADS / (2 * M);
+    const size_t lds = diag_table_lds(a, M);
+    const int64_t rgrid = std::max<int64_t>(1, std::min<int64_t>((a.P + 255) / 256, int64_t(num_cu) * (lds ? 4 : 16)));
+    hipLaunchKernelGGL((diag_rank_kernel<M>), dim3(unsigned(rgrid)), dim3(256), lds, s, a, lds ? 1 : 0);
+    int64_t start = 0;
     for (int w = 0; w <= a.S; ++w) {
         const int64_t n = counts[w] * a.nb;
-        if (n == 0) continue;
-        const int64_t blocks = (n + 255) / 256;
+        const int64_t blocks = (n + PB - 1) / PB;
         if (blocks > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "weight %d has too many params for one launch", w);
-        hipLaunchKernelGGL((diag_weight_kernel<M, REORDER>), dim3(unsigned(blocks)), dim3(256), 0, s, a, w, counts[w]);
+        if (blocks > 0)
+            hipLaunchKernelGGL((diag_group_kernel<M, REORDER>), dim3(unsigned(blocks)), dim3(DG_THREADS), 0, s, a, w, counts[w], start);
+        start += counts[w];
     }
     return MM_OK;
 }
@@ -1294,29 +1309,33 @@ int mm_hermite_diagonal(const double *A, const double *B, const double *c, int c
     a.sq = ds->sq;
     a.rsq = ds->rsq;
     char *ws = reinterpret_cast<char *>(workspace);
-    int64_t *T = reinterpret_cast<int64_t *>(ws);
-    int64_t *pre = T + int64_t(M + 1) * (a.S + 1);
-    a.T = T;
+    a.T = reinterpret_cast<int64_t *>(ws);
+    a.Q = a.T + int64_t(M + 1) * (a.S + 1);
+    a.list = reinterpret_cast<int32_t *>(ws + align256(size_t(2 * (M + 1)) * size_t(a.S + 1) * 8));
+    a.fwd = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(a.list) + align256(size_t(a.P) * 4));
     a.arr0 = reinterpret_cast<double2 *>(G);
-    a.arr1 = reinterpret_cast<double2 *>(ws + g.t_bytes);
-    a.arr2 = a.arr1 + 2 * M * a.P * nb;
+    a.inD = reinterpret_cast<double2 *>(ws + g.t_bytes);
+    a.inO = a.inD + 2 * M * a.P * nb;
+    a.arr2 = a.inO + 2 * M * a.P * nb;
     a.a1010 = a.arr2 + g.n2 * nb;
     a.a1001 = a.a1010 + g.n11 * nb;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(1024), 0, s, T, pre, a, M);
-    LAUNCH_CHECK("diag_counts_kernel");
+    if (diag_table_lds(a, M) == 0) {
+        hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(64), 0, s, a, M);
+        LAUNCH_CHECK("diag_counts_kernel");
+    }
     const std::vector<int64_t> counts = diag_weight_counts(a, M);
     switch (M * 2 + (reordered ? 1 : 0)) {
 #define CASE(MM)                                                       \
-    case 2 * MM: rc = launch_diag_weights<MM, false>(a, counts, s); break; \
-    case 2 * MM + 1: rc = launch_diag_weights<MM, true>(a, counts, s); break;
+    case 2 * MM: rc = launch_diag_weights<MM, false>(a, counts, ds->num_cu, s); break; \
+    case 2 * MM + 1: rc = launch_diag_weights<MM, true>(a, counts, ds->num_cu, s); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
 #undef CASE
         default:
             return fail(MM_ERR_INVALID, "bad M");
     }
     if (rc) return rc;
-    LAUNCH_CHECK("diag_weight_kernel");
+    LAUNCH_CHECK("diag_group_kernel");
     return MM_OK;
 }
 

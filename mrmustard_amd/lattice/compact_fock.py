@@ -20,13 +20,15 @@ from .strategies import C128, _ptr, _shape_arr, _stream, _workspace, as_device_c
 __all__ = ["diagonal_hip", "input_validation"]
 
 
-def input_validation(A: torch.Tensor, rtol: float = 1e-05, atol: float = 1e-08) -> bool:
-    """A square, NaN-free and symmetric within (rtol, atol) (inputValidation.py:25-58)."""
-    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+def input_validation(A, rtol: float = 1e-05, atol: float = 1e-08) -> bool:
+    """A square, NaN-free and symmetric within (rtol, atol) (inputValidation.py:25-58).
+    A is at most 20 x 20: one copy to the host, then numpy's own checks."""
+    Ah = A.detach().cpu().numpy() if isinstance(A, torch.Tensor) else np.asarray(A)
+    if Ah.ndim != 2 or Ah.shape[0] != Ah.shape[1]:
         raise ValueError("Input matrix must be square.")
-    if bool(torch.isnan(A).any()):
+    if np.isnan(Ah).any():
         raise ValueError("Input matrix must not contain NaNs.")
-    if not bool(torch.allclose(A, A.T, rtol=rtol, atol=atol)):
+    if not np.allclose(Ah, Ah.T, rtol=rtol, atol=atol):
         raise ValueError("Input matrix must be symmetric.")
     return True
 

@@ -178,6 +178,11 @@ def test_diagonal_hip_closed_forms():
     t = np.array([0.5, 0.75])
     want = c * np.outer(t[0] ** np.arange(100), t[1] ** np.arange(80))
     assert np.max(np.abs(got - want)) < 1e-14
+    # one mode, 2000 photon numbers: count tables too large for LDS (built in global memory)
+    A, b, c = thermal_dm([5.0])
+    got = diagonal_hip(A, b, c, (2000,), reordered=True).cpu().numpy()
+    want = c * (5.0 / 6.0) ** np.arange(2000)
+    assert np.max(np.abs(got - want)) < 1e-14
 
 
 @pytest.mark.gpu
