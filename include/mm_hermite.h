@@ -156,6 +156,25 @@ size_t mm_hermite_diagonal_workspace(const int64_t *cutoffs, int M, int64_t nb);
 int mm_hermite_diagonal(const double *A, const double *B, const double *c, int c_batched, const int64_t *cutoffs, int M,
                         int64_t nb, int reordered, double *G, void *workspace, size_t ws_bytes, void *stream);
 
+/* -------------------------------------------------------- one leftover mode
+ *
+ * The conditional density matrices of mode 0 when the Mt <= 7 other modes are
+ * PNR-detected: out[m, n, k_1..k_Mt] = G[(m, k), (n, k)] for m, n <=
+ * output_cutoff and k_j <= pnr_cutoffs[j], shape (output_cutoff + 1,
+ * output_cutoff + 1, pnr_cutoffs + 1) (device, complex128, written in full).
+ * Replaces hermite_multidimensional_1leftoverMode (mrmustard/math/lattice/
+ * strategies/compactFock/inputValidation.py:103-118 -> singleLeftoverMode_amps
+ * .py:369-421) as bound by the JAX backend (math/backend_jax.py:524-534,
+ * jax_vjps/hermite.py:343-387); the numpy backend's fast_diagonal
+ * (math/backend_numpy.py:434-446, strategies/fast_diagonal.py:32-80) returns
+ * the same amplitudes.  A (2L x 2L, L = Mt + 1), b (2L), c (1) in the
+ * [modes..., modes...] order when reordered != 0, else [mode0, mode0, mode1,
+ * mode1, ...].  Inner products are summed in index order (rounding-level
+ * differences from the reference's BLAS dot). */
+size_t mm_hermite_1leftover_workspace(int64_t output_cutoff, const int64_t *pnr_cutoffs, int Mt);
+int mm_hermite_1leftover(const double *A, const double *b, const double *c, int64_t output_cutoff, const int64_t *pnr_cutoffs,
+                         int Mt, int reordered, double *out, void *workspace, size_t ws_bytes, void *stream);
+
 /* ------------------------------------------------------ contraction (MFMA)
  *
  * C = A * B for complex128 row-major matrices (A: M x K with row stride lda,

@@ -25,6 +25,7 @@
 #include "autoshape.hpp"
 #include "binomial.hpp"
 #include "diagonal.hpp"
+#include "leftover.hpp"
 
 using namespace mm;
 
@@ -787,6 +788,48 @@ int launch_diag_weights(const DiagArgs &a, const std::vector<int64_t> &counts, i
     return MM_OK;
 }
 
+// ------------------------------------------------------------ one leftover mode
+
+constexpr int LO_MAXT = 7;  // PNR-detected modes
+
+int lo_geom(int64_t output_cutoff, const int64_t *pnr, int Mt, DiagGeom *g, int *c0, size_t *bytes) {
+    if (!pnr) return fail(MM_ERR_INVALID, "pnr_cutoffs pointer is NULL");
+    if (Mt < 1) return fail(MM_ERR_INVALID, "The number of modes should be greater than 1.");
+    if (Mt > LO_MAXT) return fail(MM_ERR_UNSUPPORTED, "1leftoverMode supports up to %d detected modes, got %d", LO_MAXT, Mt);
+    if (output_cutoff < 0 || output_cutoff + 3 >= MM_MAX_CUTOFF)
+        return fail(MM_ERR_INVALID, "output_cutoff=%lld outside [0, %d)", (long long)output_cutoff, MM_MAX_CUTOFF - 3);
+    int64_t tail[DG_MAXM];
+    for (int j = 0; j < Mt; ++j) {
+        if (pnr[j] < 0) return fail(MM_ERR_INVALID, "pnr_cutoffs[%d]=%lld is negative", j, (long long)pnr[j]);
+        tail[j] = pnr[j] + 1;
+    }
+    *c0 = int(output_cutoff + 1);
+    int rc = diag_geom(tail, Mt, int64_t(*c0) * *c0, g);
+    if (rc) return rc;
+    *bytes = g->bytes + size_t(16) * size_t(Mt) * size_t(g->a.P) * size_t(*c0) * size_t(*c0);  // + piv
+    return MM_OK;
+}
+
+template <int MT, bool REORDER>
+int launch_leftover(LeftArgs &a, const std::vector<int64_t> &counts, int num_cu, hipStream_t s) {
+    constexpr int PB = DG_THREADS / (2 * MT);
+    const size_t lds = diag_table_lds(a.t, MT);
+    const int64_t rgrid = std::max<int64_t>(1, std::min<int64_t>((a.t.P + 255) / 256, int64_t(num_cu) * (lds ? 4 : 16)));
+    hipLaunchKernelGGL((diag_rank_kernel<MT>), dim3(unsigned(rgrid)), dim3(256), lds, s, a.t, lds ? 1 : 0);
+    hipLaunchKernelGGL((lo_init_kernel<MT, REORDER>), dim3(1), dim3(64), 0, s, a);
+    int64_t start = 0;
+    for (int w = 0; w <= a.t.S; ++w) {
+        const int64_t blocks = (counts[w] * a.C + PB - 1) / PB;
+        if (blocks > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "weight %d has too many entries for one launch", w);
+        if (blocks > 0) {
+            hipLaunchKernelGGL((lo_diag_kernel<MT, REORDER>), dim3(unsigned(blocks)), dim3(DG_THREADS), 0, s, a, w, counts[w], start);
+            hipLaunchKernelGGL((lo_off_kernel<MT, REORDER>), dim3(unsigned(blocks)), dim3(DG_THREADS), 0, s, a, w, counts[w], start);
+        }
+        start += counts[w];
+    }
+    return MM_OK;
+}
+
 struct GlobalJob {
     Shape S;
     int64_t a_off, b_off, c_off, g_off;
@@ -1282,6 +1325,69 @@ int mm_autoshape(const double *A, const double *b, const double *c, int64_t B, i
     a.shape = shape;
     hipLaunchKernelGGL(autoshape_kernel, dim3(unsigned(B * M)), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), a);
     LAUNCH_CHECK("autoshape_kernel");
+    return MM_OK;
+}
+
+size_t mm_hermite_1leftover_workspace(int64_t output_cutoff, const int64_t *pnr_cutoffs, int Mt) {
+    DiagGeom g;
+    int c0 = 0;
+    size_t bytes = 0;
+    return lo_geom(output_cutoff, pnr_cutoffs, Mt, &g, &c0, &bytes) ? 0 : bytes;
+}
+
+int mm_hermite_1leftover(const double *A, const double *b, const double *c, int64_t output_cutoff, const int64_t *pnr_cutoffs,
+                         int Mt, int reordered, double *out, void *workspace, size_t ws_bytes, void *stream) {
+    DiagGeom g;
+    int c0 = 0;
+    size_t need = 0;
+    int rc = lo_geom(output_cutoff, pnr_cutoffs, Mt, &g, &c0, &need);
+    if (rc) return rc;
+    if (!A || !b || !c || !out) return fail(MM_ERR_INVALID, "NULL array pointer");
+    if (!workspace || ws_bytes < need) return fail(MM_ERR_WORKSPACE, "1leftover workspace of %zu bytes needed, %zu given", need, ws_bytes);
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    LeftArgs a;
+    a.t = g.a;
+    DiagArgs &t = a.t;
+    t.sq = ds->sq;
+    t.rsq = ds->rsq;
+    char *ws = reinterpret_cast<char *>(workspace);
+    const int64_t C = int64_t(c0) * c0;
+    t.T = reinterpret_cast<int64_t *>(ws);
+    t.Q = t.T + int64_t(Mt + 1) * (t.S + 1);
+    t.list = reinterpret_cast<int32_t *>(ws + align256(size_t(2 * (Mt + 1)) * size_t(t.S + 1) * 8));
+    t.fwd = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(t.list) + align256(size_t(t.P) * 4));
+    t.inD = reinterpret_cast<double2 *>(ws + g.t_bytes);
+    t.inO = t.inD + 2 * Mt * t.P * C;
+    t.arr2 = t.inO + 2 * Mt * t.P * C;
+    t.a1010 = t.arr2 + g.n2 * C;
+    t.a1001 = t.a1010 + g.n11 * C;
+    t.arr0 = nullptr;
+    a.piv = reinterpret_cast<double2 *>(ws + g.bytes);
+    a.A = reinterpret_cast<const double2 *>(A);
+    a.B = reinterpret_cast<const double2 *>(b);
+    a.c = reinterpret_cast<const double2 *>(c);
+    a.c0 = c0;
+    a.C = int(C);
+    a.out = reinterpret_cast<double2 *>(out);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (diag_table_lds(t, Mt) == 0) {
+        hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(64), 0, s, t, Mt);
+        LAUNCH_CHECK("diag_counts_kernel");
+    }
+    const std::vector<int64_t> counts = diag_weight_counts(t, Mt);
+    switch (Mt * 2 + (reordered ? 1 : 0)) {
+#define CASE(MM)                                                             \
+    case 2 * MM: rc = launch_leftover<MM, false>(a, counts, ds->num_cu, s); break; \
+    case 2 * MM + 1: rc = launch_leftover<MM, true>(a, counts, ds->num_cu, s); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad number of detected modes");
+    }
+    if (rc) return rc;
+    LAUNCH_CHECK("lo kernels");
     return MM_OK;
 }
 

@@ -5,6 +5,11 @@
   ``hermite_multidimensional_diagonal`` (-> diagonal_amps.py:200-248): the
   diagonal G[k0, k0, k1, k1, ...] of a 2M-index triple, shape ``cutoffs``
   (``cutoffs + (nb,)`` for a batch of B's).
+- ``leftover_hip`` <- compactFock/inputValidation.py:103-118
+  ``hermite_multidimensional_1leftoverMode`` (-> singleLeftoverMode_amps.py
+  :369-421), as called by hermite_renormalized_1leftoverMode_jax
+  (jax_vjps/hermite.py:343-387): the conditional density matrices of mode 0,
+  shape ``(output_cutoff + 1, output_cutoff + 1, *(pnr_cutoffs + 1))``.
 """
 
 from __future__ import annotations
@@ -17,7 +22,7 @@ import torch
 from .. import _lib
 from .strategies import C128, _ptr, _shape_arr, _stream, _workspace, as_device_c128
 
-__all__ = ["diagonal_hip", "input_validation"]
+__all__ = ["diagonal_hip", "input_validation", "leftover_hip"]
 
 
 def input_validation(A, rtol: float = 1e-05, atol: float = 1e-08) -> bool:
@@ -69,3 +74,37 @@ def diagonal_hip(A, B, G0, cutoffs, reordered: bool = False, rtol: float = 1e-05
     )
     _lib.check(rc, "mm_hermite_diagonal")
     return G
+
+
+def leftover_hip(A, b, c, output_cutoff: int, pnr_cutoffs, reordered: bool = True, rtol: float = 1e-05,
+                 atol: float = 1e-08) -> torch.Tensor:
+    """out[m, n, k] = G[(m, k), (n, k)]: mode 0's density matrix (m, n <=
+    ``output_cutoff``) with the other modes detected with k_j <= pnr_cutoffs[j]
+    photons.  (A, b, c) in the [modes..., modes...] order when ``reordered``
+    (reorder_AB_bargmann, backend_numpy.py:367-375), else [mode0, mode0, mode1,
+    mode1, ...].  Raises ValueError as inputValidation.py:103-118 does."""
+    A, b, c = as_device_c128(A), as_device_c128(b), as_device_c128(c)
+    input_validation(A, rtol=rtol, atol=atol)
+    if A.shape[0] != b.shape[0]:
+        raise ValueError("The matrix A and vector B have incompatible dimensions")
+    if not isinstance(pnr_cutoffs, Iterable):
+        raise ValueError("cutoffs should be array like of length M")
+    pnr = tuple(int(k) for k in pnr_cutoffs)
+    M = len(pnr) + 1
+    if A.shape[0] // 2 != M or A.shape[0] != 2 * M:
+        raise ValueError("The matrix A and cutoffs have incompatible dimensions")
+    if M <= 1:
+        raise ValueError("The number of modes should be greater than 1.")
+    if c.numel() != 1:
+        raise ValueError("c must be a scalar")
+    o = int(output_cutoff)
+    out = torch.empty((o + 1, o + 1, *(k + 1 for k in pnr)), dtype=C128, device=A.device)
+    s = _shape_arr(pnr)
+    L = _lib.load()
+    ws = _workspace(L.mm_hermite_1leftover_workspace(o, s.ctypes.data, M - 1))
+    rc = L.mm_hermite_1leftover(
+        _ptr(A), _ptr(b), _ptr(c), o, s.ctypes.data, M - 1, int(bool(reordered)),
+        _ptr(out), _ptr(ws), 0 if ws is None else ws.numel(), _stream(),
+    )
+    _lib.check(rc, "mm_hermite_1leftover")
+    return out

@@ -120,6 +120,13 @@ class BackendHip:
         the [modes..., modes...] -> [mode0, mode0, ...] reorder happens in the kernel."""
         return cf.diagonal_hip(A, B, C, cutoffs, reordered=reorderedAB)
 
+    def hermite_renormalized_1leftoverMode(self, A, b, c, output_cutoff: int, pnr_cutoffs: tuple[int, ...],
+                                           reorderedAB: bool = True):
+        """Conditional density matrices of mode 0 (backend_jax.py:524-534 ->
+        jax_vjps/hermite.py:343-387; the numpy backend's fast_diagonal,
+        backend_numpy.py:434-446, returns the same amplitudes)."""
+        return cf.leftover_hip(A, b, c, output_cutoff, pnr_cutoffs, reordered=reorderedAB)
+
     def hermite_renormalized_batched(self, A, b, c, shape: tuple[int], stable: bool = False, out=None):
         """Batched path (backend_numpy.py:394-403 -> batch.py:27)."""
         if _needs_grad(A, b, c):

@@ -165,6 +165,14 @@ class BackendManager:
         (backend_manager.py:729-757)."""
         return self._apply("hermite_renormalized_diagonal", (A, b, c, cutoffs, reorderedAB))
 
+    def hermite_renormalized_1leftoverMode(self, A, b, c, output_cutoff: int, pnr_cutoffs: tuple[int, ...],
+                                           reorderedAB: bool = True):
+        r"""The density matrix of mode 0 when all the other modes are detected
+        with PNR detectors up to the given photon numbers: shape
+        ``(output_cutoff + 1, output_cutoff + 1, *pnr_cutoffs + 1)``
+        (backend_manager.py:759-786)."""
+        return self._apply("hermite_renormalized_1leftoverMode", (A, b, c, output_cutoff, pnr_cutoffs, reorderedAB))
+
     def _as_input(self, x):
         if hasattr(x, "ndim") and hasattr(x, "shape"):
             return x

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_symbols():
     text = open(os.path.join(ROOT, "include", "mm_hermite.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(mm_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(mm_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_header_symbols():
