@@ -315,6 +315,23 @@ def main():
             "note": "API call: input validation, workspace, one launch per total photon number",
         }
 
+        # ---- one leftover mode: heralded conditional density matrices of a 3-mode density matrix
+        log("1leftoverMode: 3-mode lossy density matrix, mode 0 up to 19 photons, two PNR modes up to 14")
+        from mrmustard_amd.lattice.compact_fock import leftover_hip
+
+        Al, bl, cl = W.pnr_dm(3, seed=7)
+        Alt, blt, clt = (torch.as_tensor(np.asarray(x), device=dev) for x in (Al, bl, cl))
+        o_l, pnr_l = 19, (14, 14)
+        t_l = timed(lambda: leftover_hip(Alt, blt, clt, o_l, pnr_l), 10, 2)
+        n_l = (o_l + 1) ** 2 * int(np.prod([k + 1 for k in pnr_l]))
+        extra["leftover_3modes_20x20x15x15"] = {
+            "entries_per_s": n_l / t_l,
+            "ms_per_call": t_l * 1e3,
+            "entries": n_l,
+            "launches": 2 * (sum(pnr_l) + 1) + 2,
+            "note": "API call: input validation, workspace, two launches per total detected photon number",
+        }
+
     cpu = None
     parity = None
     if not args.no_cpu and not args.quick:
@@ -367,6 +384,18 @@ def main():
             extra["diagonal_6modes_cutoff10"]["cpu_oracle_probabilities_per_s"] = float(np.prod(cut_s)) / t_cpu_d
             extra["diagonal_6modes_cutoff10"]["cpu_oracle_sample"] = "same state at cutoffs 4 per mode, Python restatement, one core"
             extra["diagonal_6modes_cutoff10"]["rel_err_vs_oracle_on_sample"] = rel(p_gpu, p_ref)
+        if "leftover_3modes_20x20x15x15" in extra:
+            from oracle import leftover as oracle_leftover
+
+            o_s, pnr_s = 5, (4, 4)
+            t0 = time.perf_counter()
+            l_ref = oracle_leftover.leftover(Al, bl, cl, o_s, pnr_s)
+            t_cpu_l = time.perf_counter() - t0
+            l_gpu = leftover_hip(Alt, blt, clt, o_s, pnr_s).cpu().numpy()
+            ex = extra["leftover_3modes_20x20x15x15"]
+            ex["cpu_oracle_entries_per_s"] = l_ref.size / t_cpu_l
+            ex["cpu_oracle_sample"] = "same state, mode 0 up to 5 photons, PNR modes up to 4, Python restatement, one core"
+            ex["rel_err_vs_oracle_on_sample"] = rel(l_gpu, l_ref)
         log("CPU oracle baseline")
         threads = oracle.max_threads()
         triples = [(A_np[k], b_np[k], c_np[k]) for k in range(B)]

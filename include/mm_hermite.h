@@ -158,8 +158,8 @@ int mm_hermite_diagonal(const double *A, const double *B, const double *c, int c
 
 /* -------------------------------------------------------- one leftover mode
  *
- * The conditional density matrices of mode 0 when the Mt <= 7 other modes are
- * PNR-detected: out[m, n, k_1..k_Mt] = G[(m, k), (n, k)] for m, n <=
+ * The conditional density matrices of mode 0 (output_cutoff <= 1023) when the
+ * Mt <= 7 other modes are PNR-detected: out[m, n, k_1..k_Mt] = G[(m, k), (n, k)] for m, n <=
  * output_cutoff and k_j <= pnr_cutoffs[j], shape (output_cutoff + 1,
  * output_cutoff + 1, pnr_cutoffs + 1) (device, complex128, written in full).
  * Replaces hermite_multidimensional_1leftoverMode (mrmustard/math/lattice/

@@ -796,8 +796,8 @@ int lo_geom(int64_t output_cutoff, const int64_t *pnr, int Mt, DiagGeom *g, int 
     if (!pnr) return fail(MM_ERR_INVALID, "pnr_cutoffs pointer is NULL");
     if (Mt < 1) return fail(MM_ERR_INVALID, "The number of modes should be greater than 1.");
     if (Mt > LO_MAXT) return fail(MM_ERR_UNSUPPORTED, "1leftoverMode supports up to %d detected modes, got %d", LO_MAXT, Mt);
-    if (output_cutoff < 0 || output_cutoff + 3 >= MM_MAX_CUTOFF)
-        return fail(MM_ERR_INVALID, "output_cutoff=%lld outside [0, %d)", (long long)output_cutoff, MM_MAX_CUTOFF - 3);
+    if (output_cutoff < 0) return fail(MM_ERR_INVALID, "output_cutoff=%lld is negative", (long long)output_cutoff);
+    if (output_cutoff >= 1024) return fail(MM_ERR_UNSUPPORTED, "output_cutoff=%lld: at most 1023", (long long)output_cutoff);
     int64_t tail[DG_MAXM];
     for (int j = 0; j < Mt; ++j) {
         if (pnr[j] < 0) return fail(MM_ERR_INVALID, "pnr_cutoffs[%d]=%lld is negative", j, (long long)pnr[j]);
@@ -816,7 +816,8 @@ int launch_leftover(LeftArgs &a, const std::vector<int64_t> &counts, int num_cu,
     const size_t lds = diag_table_lds(a.t, MT);
     const int64_t rgrid = std::max<int64_t>(1, std::min<int64_t>((a.t.P + 255) / 256, int64_t(num_cu) * (lds ? 4 : 16)));
     hipLaunchKernelGGL((diag_rank_kernel<MT>), dim3(unsigned(rgrid)), dim3(256), lds, s, a.t, lds ? 1 : 0);
-    hipLaunchKernelGGL((lo_init_kernel<MT, REORDER>), dim3(1), dim3(64), 0, s, a);
+    const int ithreads = (a.c0 + 63) / 64 * 64;
+    hipLaunchKernelGGL((lo_init_kernel<MT, REORDER>), dim3(1), dim3(ithreads), size_t(3) * a.c0 * 16, s, a);
     int64_t start = 0;
     for (int w = 0; w <= a.t.S; ++w) {
         const int64_t blocks = (counts[w] * a.C + PB - 1) / PB;

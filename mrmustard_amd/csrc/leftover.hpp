@@ -31,33 +31,52 @@ struct LeftArgs {
 };
 
 // The leftover block at zero detections: the two-index lattice of
-// (A[:2, :2], B[:2], c) (singleLeftoverMode_amps.py:330-341), one thread.
+// (A[:2, :2], B[:2], c) (singleLeftoverMode_amps.py:330-341).  Column 0 is a
+// sequential recurrence (thread 0, in registers); column n + 1 needs columns n
+// and n - 1 only, so each later column is one step of c0 threads over three
+// rotating columns in LDS (dynamic, 3 c0 entries).  c0 <= blockDim.x.
 template <int MT, bool REORDER>
-__global__ __launch_bounds__(64) void lo_init_kernel(LeftArgs a) {
-    if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(1024) void lo_init_kernel(LeftArgs a) {
+    extern __shared__ double2 cols[];  // 3 x c0
     constexpr int L = MT + 1;
-    const int c0 = a.c0;
+    const int c0 = a.c0, tid = threadIdx.x;
     const int64_t P = a.t.P;
     const double2 A00 = a.A[src_index<L, REORDER>(0) * 2 * L + src_index<L, REORDER>(0)];
     const double2 A10 = a.A[src_index<L, REORDER>(1) * 2 * L + src_index<L, REORDER>(0)];
     const double2 A11 = a.A[src_index<L, REORDER>(1) * 2 * L + src_index<L, REORDER>(1)];
     const double2 B0 = a.B[src_index<L, REORDER>(0)], B1 = a.B[src_index<L, REORDER>(1)];
-    double2 *blk = a.t.inO;  // slot 0 of position 0
-    auto at = [&](int m, int n) -> double2 & { return blk[m * c0 + n]; };
-    at(0, 0) = a.c[0];
-    for (int m = 0; m + 1 < c0; ++m) {
-        double2 t = cmul(at(m, 0), B0);
-        if (m > 0) t = cadd(t, cmul(cscale_promoted(A00, a.t.sq[m]), at(m - 1, 0)));
-        at(m + 1, 0) = cdiv_by(t, a.t.sq[m + 1], a.t.rsq[m + 1]);
-    }
-    for (int m = 0; m < c0; ++m)
-        for (int n = 0; n + 1 < c0; ++n) {
-            double2 t = cmul(at(m, n), B1);
-            if (m > 0) t = cadd(t, cmul(cscale_promoted(A10, a.t.sq[m]), at(m - 1, n)));
-            if (n > 0) t = cadd(t, cmul(cscale_promoted(A11, a.t.sq[n]), at(m, n - 1)));
-            at(m, n + 1) = cdiv_by(t, a.t.sq[n + 1], a.t.rsq[n + 1]);
+    double2 *blk = a.t.inO;  // inbox slot 0 of position 0: entry (m, n) at m * c0 + n
+    if (tid == 0) {
+        double2 prev = make_double2(0.0, 0.0), cur = a.c[0];
+        cols[0] = cur;
+        for (int m = 0; m + 1 < c0; ++m) {
+            double2 t = cmul(cur, B0);
+            if (m > 0) t = cadd(t, cmul(cscale_promoted(A00, a.t.sq[m]), prev));
+            prev = cur;
+            cur = cdiv_by(t, a.t.sq[m + 1], a.t.rsq[m + 1]);
+            cols[m + 1] = cur;
         }
-    for (int e = 0; e < c0 * c0; ++e) a.out[int64_t(e) * P] = blk[e];
+    }
+    __syncthreads();
+    if (tid < c0) {
+        blk[tid * c0] = cols[tid];
+        a.out[int64_t(tid * c0) * P] = cols[tid];
+    }
+    for (int n = 0; n + 1 < c0; ++n) {
+        const double2 *cur = cols + (n % 3) * c0, *prv = cols + ((n + 2) % 3) * c0;
+        double2 *nxt = cols + ((n + 1) % 3) * c0;
+        if (tid < c0) {
+            const int m = tid;
+            double2 t = cmul(cur[m], B1);
+            if (m > 0) t = cadd(t, cmul(cscale_promoted(A10, a.t.sq[m]), cur[m - 1]));
+            if (n > 0) t = cadd(t, cmul(cscale_promoted(A11, a.t.sq[n]), prv[m]));
+            const double2 v = cdiv_by(t, a.t.sq[n + 1], a.t.rsq[n + 1]);
+            nxt[m] = v;
+            blk[m * c0 + n + 1] = v;
+            a.out[int64_t(m * c0 + n + 1) * P] = v;
+        }
+        __syncthreads();
+    }
 }
 
 // Shared set-up of both launches: lane (slot, i), group = (params, block entry).

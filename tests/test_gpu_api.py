@@ -303,6 +303,22 @@ def test_shim_backend_numbers(oracle_mod):
     Ab, bb, cb = random_triple(2, (3,), seed=22)
     Gb = m._backend.hermite_renormalized_batched(Ab, bb, cb, (4, 5))
     assert np.array_equal(Gb, oracle_mod.vanilla_batch((4, 5), Ab, bb, cb))
+    # the strategies beside the vanilla path, with the numpy backend's signatures
+    from oracle import binomial as OB
+    from oracle import diagonal as OD
+    from oracle import leftover as OL
+
+    from tests.test_diagonal import random_dm
+
+    Gbin = m._backend.hermite_renormalized_binomial(A, b, c, (5, 4, 6), 1e9, None)
+    assert isinstance(Gbin, np.ndarray) and np.array_equal(Gbin, OB.binomial((5, 4, 6), A, b, c, 1e9, 13)[0])
+    Ad, bd, cd = random_dm(2, 23)
+    pd = m._backend.hermite_renormalized_diagonal(Ad, bd, cd, (4, 5), True)
+    ref = OD.diagonal(*OD.reorder_ab(Ad, bd), cd, (4, 5))
+    assert isinstance(pd, np.ndarray) and np.max(np.abs(pd - ref)) <= 1e-12 * np.max(np.abs(ref))
+    pl = m._backend.hermite_renormalized_1leftoverMode(Ad, bd, cd, 3, (4,), True)
+    ref = OL.leftover(Ad, bd, cd, 3, (4,))
+    assert pl.shape == (4, 4, 5) and np.max(np.abs(pl - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
 # ---------------------------------------------------------------- contraction

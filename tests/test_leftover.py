@@ -116,6 +116,10 @@ def test_leftover_hip_product_state():
     alpha, nbars, o, pnr = 1.3 + 0.4j, (0.5, 2.0), 20, (30, 25)
     got = leftover_hip(*product_state(alpha, nbars), o, pnr).cpu().numpy()
     assert np.max(np.abs(got - product_closed_form(alpha, nbars, o, pnr))) < 1e-14
+    # a leftover block wider than one wave (101 x 101)
+    alpha, nbars, o, pnr = 2.5 - 1.0j, (1.0,), 100, (3,)
+    got = leftover_hip(*product_state(alpha, nbars), o, pnr).cpu().numpy()
+    assert np.max(np.abs(got - product_closed_form(alpha, nbars, o, pnr))) < 1e-14
 
 
 @pytest.mark.gpu
