@@ -156,6 +156,20 @@ size_t mm_hermite_diagonal_workspace(const int64_t *cutoffs, int M, int64_t nb);
 int mm_hermite_diagonal(const double *A, const double *B, const double *c, int c_batched, const int64_t *cutoffs, int M,
                         int64_t nb, int reordered, double *G, void *workspace, size_t ws_bytes, void *stream);
 
+/* Forward and vector-Jacobian product of the diagonal strategy in one call
+ * (unbatched B): G as mm_hermite_diagonal, then dA (2M x 2M), dB (2M), dc (1)
+ * = sum_k dLdG[k] dG[k]/d(A, B, c), holomorphic (dLdG not conjugated), every
+ * entry of A independent (not symmetrised), in the caller's index order.
+ * Replaces fock_representation_diagonal_grad (compactFock/diagonal_grad.py
+ * :19-38, :41-258, :357-384) contracted with the cotangent as in
+ * hermite_renormalized_diagonal_jax_bwd (jax_vjps/hermite.py:292-330); that
+ * forward-mode Jacobian of prod(cutoffs) x (2M)^2 entries is never formed.
+ * Sums are accumulated in a fixed order (deterministic run to run). */
+size_t mm_hermite_diagonal_vjp_workspace(const int64_t *cutoffs, int M);
+int mm_hermite_diagonal_vjp(const double *A, const double *B, const double *c, const int64_t *cutoffs, int M, int reordered,
+                            const double *dLdG, double *G, double *dA, double *dB, double *dc, void *workspace, size_t ws_bytes,
+                            void *stream);
+
 /* -------------------------------------------------------- one leftover mode
  *
  * The conditional density matrices of mode 0 (output_cutoff <= 1023) when the

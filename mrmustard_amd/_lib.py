@@ -50,6 +50,8 @@ SIGNATURES = {
     "mm_autoshape": (_int, [_vp, _vp, _vp, _i64, _int, ctypes.c_double, _int, _int, _vp, _vp]),
     "mm_hermite_diagonal_workspace": (_sz, [_vp, _int, _i64]),
     "mm_hermite_diagonal": (_int, [_vp, _vp, _vp, _int, _vp, _int, _i64, _int, _vp, _vp, _sz, _vp]),
+    "mm_hermite_diagonal_vjp_workspace": (_sz, [_vp, _int]),
+    "mm_hermite_diagonal_vjp": (_int, [_vp, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mm_hermite_1leftover_workspace": (_sz, [_i64, _vp, _int]),
     "mm_hermite_1leftover": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _int, _vp, _vp, _sz, _vp]),
     "mm_plan_create": (_int, [_i64, _vp, _vp, _int, _vp, _vp]),

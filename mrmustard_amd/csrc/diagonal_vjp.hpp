@@ -1,0 +1,262 @@
+// Vector-Jacobian product of the diagonal strategy: (dA, dB, dc) = sum_p
+// cot[p] d arr0[p] / d(A, B, c), holomorphic (no conjugation), A's entries
+// independent, in the compact order, as fock_representation_diagonal_grad
+// (compactFock/diagonal_grad.py:19-38, :41-258) differentiates the recursion
+// and hermite_renormalized_diagonal_jax_bwd (jax_vjps/hermite.py:292-330)
+// contracts it with the cotangent.
+//
+// Reverse sweep over the weights, highest first, after the forward
+// (diag_group_kernel) has filled the inboxes.  Every pivot output is
+// o_i = (g B_i + sum_l A_il x_l) / K_i with x_l = K_l[l] raw_l, so with
+// w_i = o-bar_i / K_i:
+//   dA_il += w_i x_l,  dB_i += w_i g,  g-bar += sum_i w_i B_i,
+//   raw-bar_l = K_l[l] sum_i w_i A_il.
+// A value's one consumer writes the adjoint of each input it read at the
+// location it read it from (its own inbox slot, or the d >= 1 sub-lattice
+// entry), and the producer reads the adjoints of its outputs back at the
+// locations it wrote them to: the adjoint arrays mirror the forward's.  Lane
+// i owns row i of dA and dB_i; each block keeps its sums in registers over a
+// grid-stride loop and adds them to its own partial row (the same block slot
+// in every launch), summed in block order at the end: deterministic.
+#pragma once
+
+#include "diagonal.hpp"
+
+namespace mm {
+
+struct DiagVjpArgs {
+    DiagArgs a;                        // forward geometry and stored values (nb == 1)
+    const double2 *cot;                // cotangent of arr0, by flat p
+    double2 *adD, *adO;                // adjoints of the inboxes
+    double2 *ad2, *ad1010, *ad1001;    // adjoints of the d >= 1 sub-lattices
+    double2 *part;                     // nblk x 2M x (2M + 1): dA rows, then dB
+    double2 *dc;                       // adjoint of c
+};
+
+template <int M, bool REORDER>
+__global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int w, int64_t count, int64_t start, int iters) {
+    constexpr int G = 2 * M;
+    constexpr int PB = DG_THREADS / G;
+    __shared__ double2 sA[G][G];
+    __shared__ double2 sB[G];
+    __shared__ double2 sx[G][PB + 1];
+    __shared__ double2 sw[G][PB + 1];
+    __shared__ double2 sg[PB];
+    const DiagArgs &a = v.a;
+    const int t = threadIdx.x;
+    for (int e = t; e < G * G; e += DG_THREADS) sA[e / G][e % G] = a.A[src_index<M, REORDER>(e / G) * G + src_index<M, REORDER>(e % G)];
+    if (t < G) sB[t] = a.B[src_index<M, REORDER>(t)];
+    const int slot = t % PB, i = t / PB, m = i >> 1;
+    const bool lane_ok = i < G;
+    const int64_t P = a.P;
+    double2 acc[G];
+#pragma unroll
+    for (int l = 0; l < G; ++l) acc[l] = make_double2(0.0, 0.0);
+    double2 accB = make_double2(0.0, 0.0);
+    __syncthreads();
+
+    for (int it = 0; it < iters; ++it) {
+        const int64_t grp = (int64_t(it) * gridDim.x + blockIdx.x) * PB + slot;
+        const bool act = lane_ok && grp < count;
+        int64_t f = 0, gs = 0, fw = -1;
+        int p[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) p[j] = 0;
+        if (act) {
+            gs = start + grp;
+            f = a.list[gs];
+            fw = a.fwd[m * P + gs];
+            uint32_t r = uint32_t(f);
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                const uint32_t sj = uint32_t(a.stride[j]);
+                p[j] = int(r / sj);
+                r -= uint32_t(p[j]) * sj;
+            }
+        }
+        int pm = 0, cm = 1;
+        int64_t sm = 0;
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j == m) {
+                pm = p[j];
+                cm = a.cut[j];
+                sm = a.stride[j];
+            }
+        int runmask = 0;
+        if (act) {
+            bool zp = true;
+#pragma unroll
+            for (int d = 0; d < M; ++d) {
+                if (zp && p[d] < a.cut[d] - 1) runmask |= 1 << d;
+                zp = zp && p[d] == 0;
+            }
+        }
+        const double2 g0 = act ? (w == 0 ? a.c[0] : a.inO[gs]) : make_double2(0.0, 0.0);
+        double2 gbar0 = make_double2(0.0, 0.0);  // this lane's share of arr0[p]'s adjoint
+        double2 gext = make_double2(0.0, 0.0);   // adjoint of arr1[i, p] from the off-diagonal pivot i / 2
+
+        // ---- off-diagonal pivots (diagonal_amps.py:19-105 in reverse)
+#pragma unroll
+        for (int d = 0; d < M; ++d) {
+            const bool run = (runmask >> d & 1) && i >= 2 * d;
+            double2 x = make_double2(0.0, 0.0), wv = make_double2(0.0, 0.0);
+            if (run) {
+                const int pd = p[d];
+                if (i == 2 * d) {
+                    x = cscale_promoted(g0, a.sq[pd + 1]);
+                    sg[slot] = a.inD[(2 * d + 1) * P + fw];  // arr1[2d, p], sent to p + e_d's slot 2d + 1
+                    if (pd + 2 < a.cut[d]) {
+                        const double2 ob = d == 0 ? v.adO[P + fw] : v.ad2[a.off2[d] + f];
+                        wv = cdiv_by(ob, a.sq[pd + 2], a.rsq[pd + 2]);
+                    }
+                } else if (i == 2 * d + 1) {
+                    if (pd > 0) x = cscale_promoted(d == 0 ? a.inO[P + gs] : a.arr2[a.off2[d] + f - sm], a.sq[pd]);
+                    wv = cdiv_by(v.adO[fw], a.sq[pd + 1], a.rsq[pd + 1]);  // arr0[p + e_d]: slot 0 of its consumer
+                } else {
+                    const int64_t sl = a.off11[d] + (m - d - 1) * a.len[d];
+                    if (pm > 0) {
+                        double2 xr;
+                        if (d == 0)
+                            xr = a.inO[i * P + gs];
+                        else
+                            xr = (i & 1) ? a.a1010[sl + f - sm] : a.a1001[sl + f - sm];
+                        x = cscale_promoted(xr, a.sq[pm]);
+                    }
+                    if (pm + 1 < cm) {
+                        double2 ob;
+                        if (d == 0)
+                            ob = v.adO[(i ^ 1) * P + fw];
+                        else
+                            ob = (i & 1) ? v.ad1001[sl + f] : v.ad1010[sl + f];
+                        wv = cdiv_by(ob, a.sq[pm + 1], a.rsq[pm + 1]);
+                    }
+                }
+            }
+            if (lane_ok) {
+                sx[i][slot] = x;
+                sw[i][slot] = wv;
+            }
+            __syncthreads();
+            if (run) {
+#pragma unroll
+                for (int l = 2 * d; l < G; ++l) acc[l] = cfma(wv, sx[l][slot], acc[l]);
+                accB = cfma(wv, sg[slot], accB);
+                // lane i as column l = i: sum_i' w_i' A_i'l, times K_l[l]
+                double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int r = 2 * d; r < G; ++r) s = cfma(sw[r][slot], sA[r][i], s);
+                const int pd = p[d];
+                if (i == 2 * d) {
+                    gbar0 = caxpy(a.sq[pd + 1], s, gbar0);
+                    double2 gb = make_double2(0.0, 0.0);  // adjoint of arr1[2d, p]
+#pragma unroll
+                    for (int r = 2 * d; r < G; ++r) gb = cfma(sw[r][slot], sB[r], gb);
+                    gext = gb;
+                } else if (i == 2 * d + 1) {
+                    if (pd > 0) {
+                        const double2 raw = make_double2(s.x * a.sq[pd], s.y * a.sq[pd]);
+                        if (d == 0)
+                            v.adO[P + gs] = raw;
+                        else
+                            v.ad2[a.off2[d] + f - sm] = raw;
+                    }
+                } else if (pm > 0) {
+                    const double2 raw = make_double2(s.x * a.sq[pm], s.y * a.sq[pm]);
+                    if (d == 0) {
+                        v.adO[i * P + gs] = raw;
+                    } else {
+                        const int64_t en = a.off11[d] + (m - d - 1) * a.len[d] + f - sm;
+                        if (i & 1)
+                            v.ad1010[en] = raw;
+                        else
+                            v.ad1001[en] = raw;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- diagonal pivot (diagonal_amps.py:108-150 in reverse)
+        const bool diag = act && (a.cut[0] == 1 || p[0] < a.cut[0] - 1);
+        double2 x = make_double2(0.0, 0.0), wv = make_double2(0.0, 0.0);
+        if (diag) {
+            if (pm > 0) x = cscale_promoted(a.inD[i * P + gs], a.sq[pm]);
+            if (pm + 1 < cm && (i != 1 || p[0] + 2 < a.cut[0])) {
+                // the consumer p + e_m read it iff its own diagonal pivot ran
+                const int c0p = p[0] + (m == 0 ? 1 : 0);
+                double2 ob = (a.cut[0] == 1 || c0p < a.cut[0] - 1) ? v.adD[(i ^ 1) * P + fw] : make_double2(0.0, 0.0);
+                if (!(i & 1) && (runmask >> m & 1)) ob = cadd(ob, gext);
+                wv = cdiv_by(ob, a.sq[pm + 1], a.rsq[pm + 1]);
+            }
+        }
+        if (lane_ok) {
+            sx[i][slot] = x;
+            sw[i][slot] = wv;
+        }
+        __syncthreads();
+        double2 gdiag = make_double2(0.0, 0.0);
+        if (diag) {
+#pragma unroll
+            for (int l = 0; l < G; ++l) acc[l] = cfma(wv, sx[l][slot], acc[l]);
+            accB = cfma(wv, g0, accB);
+            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int r = 0; r < G; ++r) s = cfma(sw[r][slot], sA[r][i], s);
+            if (pm > 0) v.adD[i * P + gs] = make_double2(s.x * a.sq[pm], s.y * a.sq[pm]);
+            if (i == 0) {
+#pragma unroll
+                for (int r = 0; r < G; ++r) gdiag = cfma(sw[r][slot], sB[r], gdiag);
+            }
+        }
+        // arr0[p]'s adjoint: cotangent + diagonal pivot's g term + the off-diagonal pivots' x_2d terms
+        if (lane_ok) sx[i][slot] = gbar0;
+        __syncthreads();
+        if (act && i == 0) {
+            double2 gb = cadd(v.cot[f], gdiag);
+#pragma unroll
+            for (int d = 0; d < M; ++d) gb = cadd(gb, sx[2 * d][slot]);
+            if (w == 0)
+                *v.dc = gb;
+            else
+                v.adO[gs] = gb;
+        }
+        __syncthreads();
+    }
+
+    // per-block sums over the PB params slots of each row, added to this block's partial row
+    double2 *prow = v.part + int64_t(blockIdx.x) * G * (G + 1);
+#pragma unroll
+    for (int l = 0; l <= G; ++l) {
+        if (lane_ok) sx[i][slot] = l < G ? acc[l] : accB;
+        __syncthreads();
+        if (t < G) {
+            double2 s = prow[t * (G + 1) + l];
+            for (int k = 0; k < PB; ++k) s = cadd(s, sx[t][k]);
+            prow[t * (G + 1) + l] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// dA (caller's order), dB: partial rows summed in block order.
+template <int M, bool REORDER>
+__global__ __launch_bounds__(256) void diag_vjp_finalize(const double2 *part, int nblk, double2 *dA, double2 *dB) {
+    constexpr int G = 2 * M;
+    for (int e = threadIdx.x; e < G * (G + 1); e += blockDim.x) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int b = 0; b < nblk; ++b) s = cadd(s, part[int64_t(b) * G * (G + 1) + e]);
+        const int r = e / (G + 1), l = e % (G + 1);
+        if (l < G)
+            dA[src_index<M, REORDER>(r) * G + src_index<M, REORDER>(l)] = s;
+        else
+            dB[src_index<M, REORDER>(r)] = s;
+    }
+}
+
+__global__ void zero_c128_span_kernel(double2 *p, int64_t n) {
+    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x)
+        p[k] = make_double2(0.0, 0.0);
+}
+
+}  // namespace mm

@@ -25,6 +25,7 @@
 #include "autoshape.hpp"
 #include "binomial.hpp"
 #include "diagonal.hpp"
+#include "diagonal_vjp.hpp"
 #include "leftover.hpp"
 
 using namespace mm;
@@ -788,6 +789,83 @@ int launch_diag_weights(const DiagArgs &a, const std::vector<int64_t> &counts, i
     return MM_OK;
 }
 
+// Device pointers of the forward's workspace: T and Q, list, fwd, inboxes, d >= 1 sub-lattices.
+void diag_bind(DiagGeom &g, const double *A, const double *B, const double *c, int c_batched, double *G, char *ws,
+               const DevState *ds) {
+    DiagArgs &a = g.a;
+    const int M = g.M;
+    const int64_t nb = a.nb;
+    a.A = reinterpret_cast<const double2 *>(A);
+    a.B = reinterpret_cast<const double2 *>(B);
+    a.c = reinterpret_cast<const double2 *>(c);
+    a.c_batched = c_batched ? 1 : 0;
+    a.sq = ds->sq;
+    a.rsq = ds->rsq;
+    a.T = reinterpret_cast<int64_t *>(ws);
+    a.Q = a.T + int64_t(M + 1) * (a.S + 1);
+    a.list = reinterpret_cast<int32_t *>(ws + align256(size_t(2 * (M + 1)) * size_t(a.S + 1) * 8));
+    a.fwd = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(a.list) + align256(size_t(a.P) * 4));
+    a.arr0 = reinterpret_cast<double2 *>(G);
+    a.inD = reinterpret_cast<double2 *>(ws + g.t_bytes);
+    a.inO = a.inD + 2 * M * a.P * nb;
+    a.arr2 = a.inO + 2 * M * a.P * nb;
+    a.a1010 = a.arr2 + g.n2 * nb;
+    a.a1001 = a.a1010 + g.n11 * nb;
+}
+
+int diag_forward(DiagGeom &g, int reordered, const DevState *ds, hipStream_t s, std::vector<int64_t> *counts_out) {
+    DiagArgs &a = g.a;
+    const int M = g.M;
+    if (diag_table_lds(a, M) == 0) {
+        hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(64), 0, s, a, M);
+        LAUNCH_CHECK("diag_counts_kernel");
+    }
+    const std::vector<int64_t> counts = diag_weight_counts(a, M);
+    int rc = MM_OK;
+    switch (M * 2 + (reordered ? 1 : 0)) {
+#define CASE(MM)                                                       \
+    case 2 * MM: rc = launch_diag_weights<MM, false>(a, counts, ds->num_cu, s); break; \
+    case 2 * MM + 1: rc = launch_diag_weights<MM, true>(a, counts, ds->num_cu, s); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad M");
+    }
+    if (rc) return rc;
+    LAUNCH_CHECK("diag_group_kernel");
+    if (counts_out) *counts_out = counts;
+    return MM_OK;
+}
+
+// The VJP's blocks: one per PB params of the largest weight, at most 1024.
+int diag_vjp_blocks(const DiagGeom &g) {
+    const std::vector<int64_t> counts = diag_weight_counts(g.a, g.M);
+    const int64_t mx = *std::max_element(counts.begin(), counts.end());
+    const int PB = DG_THREADS / (2 * g.M);
+    return int(std::max<int64_t>(1, std::min<int64_t>(1024, (mx + PB - 1) / PB)));
+}
+
+size_t diag_vjp_bytes(const DiagGeom &g, int nblk) {
+    const int G = 2 * g.M;
+    const size_t mirror = size_t(16) * size_t(2 * G * g.a.P + g.n2 + 2 * g.n11);
+    return align256(g.bytes) + align256(mirror) + size_t(16) * size_t(nblk) * size_t(G) * size_t(G + 1);
+}
+
+template <int M, bool REORDER>
+void launch_diag_vjp(DiagVjpArgs &v, const std::vector<int64_t> &counts, int nblk, double2 *dA, double2 *dB, hipStream_t s) {
+    constexpr int PB = DG_THREADS / (2 * M);
+    const int64_t np = int64_t(nblk) * 2 * M * (2 * M + 1);
+    hipLaunchKernelGGL(zero_c128_span_kernel, dim3(64), dim3(256), 0, s, v.part, np);
+    std::vector<int64_t> starts(counts.size(), 0);
+    for (size_t w = 1; w < counts.size(); ++w) starts[w] = starts[w - 1] + counts[w - 1];
+    const int64_t per = int64_t(nblk) * PB;
+    for (int w = v.a.S; w >= 0; --w) {
+        const int iters = int((counts[w] + per - 1) / per);
+        hipLaunchKernelGGL((diag_vjp_kernel<M, REORDER>), dim3(nblk), dim3(DG_THREADS), 0, s, v, w, counts[w], starts[w], iters);
+    }
+    hipLaunchKernelGGL((diag_vjp_finalize<M, REORDER>), dim3(1), dim3(256), 0, s, v.part, nblk, dA, dB);
+}
+
 // ------------------------------------------------------------ one leftover mode
 
 constexpr int LO_MAXT = 7;  // PNR-detected modes
@@ -1408,41 +1486,56 @@ int mm_hermite_diagonal(const double *A, const double *B, const double *c, int c
     DevState *ds = nullptr;
     rc = device_state(&ds);
     if (rc) return rc;
-    DiagArgs &a = g.a;
-    a.A = reinterpret_cast<const double2 *>(A);
-    a.B = reinterpret_cast<const double2 *>(B);
-    a.c = reinterpret_cast<const double2 *>(c);
-    a.c_batched = c_batched ? 1 : 0;
-    a.sq = ds->sq;
-    a.rsq = ds->rsq;
+    diag_bind(g, A, B, c, c_batched, G, reinterpret_cast<char *>(workspace), ds);
+    return diag_forward(g, reordered, ds, reinterpret_cast<hipStream_t>(stream), nullptr);
+}
+
+size_t mm_hermite_diagonal_vjp_workspace(const int64_t *cutoffs, int M) {
+    DiagGeom g;
+    if (diag_geom(cutoffs, M, 1, &g)) return 0;
+    return diag_vjp_bytes(g, diag_vjp_blocks(g));
+}
+
+int mm_hermite_diagonal_vjp(const double *A, const double *B, const double *c, const int64_t *cutoffs, int M, int reordered,
+                            const double *dLdG, double *G, double *dA, double *dB, double *dc, void *workspace, size_t ws_bytes,
+                            void *stream) {
+    DiagGeom g;
+    int rc = diag_geom(cutoffs, M, 1, &g);
+    if (rc) return rc;
+    if (!A || !B || !c || !dLdG || !G || !dA || !dB || !dc) return fail(MM_ERR_INVALID, "NULL array pointer");
+    const int nblk = diag_vjp_blocks(g);
+    const size_t need = diag_vjp_bytes(g, nblk);
+    if (!workspace || ws_bytes < need) return fail(MM_ERR_WORKSPACE, "diagonal VJP workspace of %zu bytes needed, %zu given", need, ws_bytes);
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
     char *ws = reinterpret_cast<char *>(workspace);
-    a.T = reinterpret_cast<int64_t *>(ws);
-    a.Q = a.T + int64_t(M + 1) * (a.S + 1);
-    a.list = reinterpret_cast<int32_t *>(ws + align256(size_t(2 * (M + 1)) * size_t(a.S + 1) * 8));
-    a.fwd = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(a.list) + align256(size_t(a.P) * 4));
-    a.arr0 = reinterpret_cast<double2 *>(G);
-    a.inD = reinterpret_cast<double2 *>(ws + g.t_bytes);
-    a.inO = a.inD + 2 * M * a.P * nb;
-    a.arr2 = a.inO + 2 * M * a.P * nb;
-    a.a1010 = a.arr2 + g.n2 * nb;
-    a.a1001 = a.a1010 + g.n11 * nb;
+    diag_bind(g, A, B, c, 0, G, ws, ds);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (diag_table_lds(a, M) == 0) {
-        hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(64), 0, s, a, M);
-        LAUNCH_CHECK("diag_counts_kernel");
-    }
-    const std::vector<int64_t> counts = diag_weight_counts(a, M);
+    std::vector<int64_t> counts;
+    rc = diag_forward(g, reordered, ds, s, &counts);
+    if (rc) return rc;
+    DiagVjpArgs v;
+    v.a = g.a;
+    v.cot = reinterpret_cast<const double2 *>(dLdG);
+    v.adD = reinterpret_cast<double2 *>(ws + align256(g.bytes));
+    v.adO = v.adD + 2 * M * g.a.P;
+    v.ad2 = v.adO + 2 * M * g.a.P;
+    v.ad1010 = v.ad2 + g.n2;
+    v.ad1001 = v.ad1010 + g.n11;
+    v.part = reinterpret_cast<double2 *>(ws + align256(g.bytes) + align256(size_t(16) * size_t(4 * M * g.a.P + g.n2 + 2 * g.n11)));
+    v.dc = reinterpret_cast<double2 *>(dc);
+    double2 *dA2 = reinterpret_cast<double2 *>(dA), *dB2 = reinterpret_cast<double2 *>(dB);
     switch (M * 2 + (reordered ? 1 : 0)) {
-#define CASE(MM)                                                       \
-    case 2 * MM: rc = launch_diag_weights<MM, false>(a, counts, ds->num_cu, s); break; \
-    case 2 * MM + 1: rc = launch_diag_weights<MM, true>(a, counts, ds->num_cu, s); break;
+#define CASE(MM)                                                                  \
+    case 2 * MM: launch_diag_vjp<MM, false>(v, counts, nblk, dA2, dB2, s); break; \
+    case 2 * MM + 1: launch_diag_vjp<MM, true>(v, counts, nblk, dA2, dB2, s); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
 #undef CASE
         default:
             return fail(MM_ERR_INVALID, "bad M");
     }
-    if (rc) return rc;
-    LAUNCH_CHECK("diag_group_kernel");
+    LAUNCH_CHECK("diag_vjp_kernel");
     return MM_OK;
 }
 

@@ -22,7 +22,7 @@ import torch
 from .. import _lib
 from .strategies import C128, _ptr, _shape_arr, _stream, _workspace, as_device_c128
 
-__all__ = ["diagonal_hip", "input_validation", "leftover_hip"]
+__all__ = ["diagonal_hip", "diagonal_vjp_hip", "input_validation", "leftover_hip"]
 
 
 def input_validation(A, rtol: float = 1e-05, atol: float = 1e-08) -> bool:
@@ -74,6 +74,36 @@ def diagonal_hip(A, B, G0, cutoffs, reordered: bool = False, rtol: float = 1e-05
     )
     _lib.check(rc, "mm_hermite_diagonal")
     return G
+
+
+def diagonal_vjp_hip(A, B, G0, cutoffs, dLdG, reordered: bool = False):
+    """(G, dA, dB, dG0): the diagonal strategy's forward and its holomorphic VJP
+    for the cotangent ``dLdG`` (shape ``cutoffs``, not conjugated), every entry
+    of A independent, in the order A and B are given in.  The reference forms
+    the Jacobians (compactFock/diagonal_grad.py) and contracts them with the
+    cotangent (jax_vjps/hermite.py:292-330); this computes the contraction
+    directly with a reverse sweep.  Unbatched B only."""
+    A, B, G0 = as_device_c128(A), as_device_c128(B), as_device_c128(G0)
+    cutoffs = tuple(int(k) for k in cutoffs)
+    M = len(cutoffs)
+    if tuple(A.shape) != (2 * M, 2 * M) or tuple(B.shape) != (2 * M,) or G0.numel() != 1:
+        raise ValueError(f"A must be ({2 * M}, {2 * M}), B ({2 * M},) and G0 scalar for cutoffs {cutoffs}")
+    g = as_device_c128(dLdG)
+    if tuple(g.shape) != cutoffs:
+        raise ValueError(f"dLdG has shape {tuple(g.shape)}, expected {cutoffs}")
+    G = torch.empty(cutoffs, dtype=C128, device=A.device)
+    dA = torch.empty((2 * M, 2 * M), dtype=C128, device=A.device)
+    dB = torch.empty((2 * M,), dtype=C128, device=A.device)
+    dc = torch.empty((), dtype=C128, device=A.device)
+    s = _shape_arr(cutoffs)
+    L = _lib.load()
+    ws = _workspace(L.mm_hermite_diagonal_vjp_workspace(s.ctypes.data, M))
+    rc = L.mm_hermite_diagonal_vjp(
+        _ptr(A), _ptr(B), _ptr(G0), s.ctypes.data, M, int(bool(reordered)), _ptr(g), _ptr(G), _ptr(dA), _ptr(dB), _ptr(dc),
+        _ptr(ws), 0 if ws is None else ws.numel(), _stream(),
+    )
+    _lib.check(rc, "mm_hermite_diagonal_vjp")
+    return G, dA, dB, dc
 
 
 def leftover_hip(A, b, c, output_cutoff: int, pnr_cutoffs, reordered: bool = True, rtol: float = 1e-05,

@@ -89,3 +89,33 @@ class HermiteRenormalizedRecompute(torch.autograd.Function):
             None,
             None,
         )
+
+
+class HermiteDiagonal(torch.autograd.Function):
+    """G = hermite_renormalized_diagonal(A, B, c, cutoffs, reorderedAB) with the
+    reverse-sweep VJP (the reference's jax custom VJP pair,
+    jax_vjps/hermite.py:248-336).  The backward recomputes the forward together
+    with the VJP (``diagonal_vjp_hip``), so only (A, B, c) are kept."""
+
+    @staticmethod
+    def forward(ctx, A, B, c, cutoffs, reordered):
+        from ..lattice import compact_fock as cf
+
+        G = cf.diagonal_hip(A, B, c, cutoffs, reordered=reordered)
+        ctx.save_for_backward(A, B, c)
+        ctx.cutoffs, ctx.reordered = tuple(cutoffs), bool(reordered)
+        return G
+
+    @staticmethod
+    def backward(ctx, grad):
+        from ..lattice import compact_fock as cf
+
+        A, B, c = ctx.saved_tensors
+        _, dA, dB, dc = cf.diagonal_vjp_hip(A, B, c, ctx.cutoffs, grad.conj().resolve_conj(), reordered=ctx.reordered)
+        return (
+            dA.conj().resolve_conj().reshape(A.shape),
+            dB.conj().resolve_conj().reshape(B.shape),
+            dc.conj().resolve_conj().reshape(c.shape),
+            None,
+            None,
+        )

@@ -24,7 +24,7 @@ import torch
 from ..lattice import compact_fock as cf
 from ..lattice import strategies as st
 from ..settings import settings
-from .autograd import HermiteRenormalized, HermiteRenormalizedBatched, HermiteRenormalizedRecompute
+from .autograd import HermiteDiagonal, HermiteRenormalized, HermiteRenormalizedBatched, HermiteRenormalizedRecompute
 
 
 def _needs_grad(*xs) -> bool:
@@ -117,7 +117,15 @@ class BackendHip:
 
     def hermite_renormalized_diagonal(self, A, B, C, cutoffs: tuple[int], reorderedAB: bool):
         """Photon-number diagonal (backend_numpy.py:423-432 -> inputValidation.py:61-79);
-        the [modes..., modes...] -> [mode0, mode0, ...] reorder happens in the kernel."""
+        the [modes..., modes...] -> [mode0, mode0, ...] reorder happens in the kernel.
+        With gradients tracked (unbatched B) it goes through the autograd
+        Function, as the JAX backend's custom VJP does (backend_jax.py:513-522)."""
+        if _needs_grad(A, B, C):
+            A, B, C = st.as_device_c128(A), st.as_device_c128(B), st.as_device_c128(C)
+            if B.ndim != 1:
+                raise ValueError("gradients of hermite_renormalized_diagonal need an unbatched B")
+            cf.input_validation(A)
+            return HermiteDiagonal.apply(A, B, C, tuple(int(k) for k in cutoffs), bool(reorderedAB))
         return cf.diagonal_hip(A, B, C, cutoffs, reordered=reorderedAB)
 
     def hermite_renormalized_1leftoverMode(self, A, b, c, output_cutoff: int, pnr_cutoffs: tuple[int, ...],
@@ -125,6 +133,8 @@ class BackendHip:
         """Conditional density matrices of mode 0 (backend_jax.py:524-534 ->
         jax_vjps/hermite.py:343-387; the numpy backend's fast_diagonal,
         backend_numpy.py:434-446, returns the same amplitudes)."""
+        if _needs_grad(A, b, c):
+            raise NotImplementedError("gradients of hermite_renormalized_1leftoverMode are not implemented on the GPU")
         return cf.leftover_hip(A, b, c, output_cutoff, pnr_cutoffs, reordered=reorderedAB)
 
     def hermite_renormalized_batched(self, A, b, c, shape: tuple[int], stable: bool = False, out=None):
