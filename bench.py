@@ -304,6 +304,10 @@ def main():
         t_d = timed(lambda: diagonal_hip(Adt, bdt, cdt, cut_d, reordered=True), 10, 2)
         p_d = diagonal_hip(Adt, bdt, cdt, cut_d, reordered=True)
         alg_d = 32 * diag_values_written(cut_d)
+        from mrmustard_amd.lattice.compact_fock import diagonal_vjp_hip
+
+        cot_d = torch.ones(cut_d, dtype=torch.complex128, device=dev)
+        t_dv = timed(lambda: diagonal_vjp_hip(Adt, bdt, cdt, cut_d, cot_d, reordered=True), 5, 1)
         extra["diagonal_6modes_cutoff10"] = {
             "probabilities_per_s": float(np.prod(cut_d)) / t_d,
             "ms_per_call": t_d * 1e3,
@@ -312,7 +316,9 @@ def main():
             "achieved_GB_per_s": alg_d / t_d / 1e9,
             "hbm_frac": alg_d / t_d / 1e9 / HBM_PEAK_GBS,
             "sum_of_probabilities": float(p_d.real.sum()),
-            "note": "API call: input validation, workspace, one launch per total photon number",
+            "fwd_plus_vjp_ms_per_call": t_dv * 1e3,
+            "note": "API call: input validation, workspace, one launch per total photon number; the VJP call "
+                    "runs the forward again, then one reverse launch per weight",
         }
 
         # ---- one leftover mode: heralded conditional density matrices of a 3-mode density matrix
