@@ -202,6 +202,14 @@ int mm_hermite_1leftover(const double *A, const double *b, const double *c, int6
 int mm_zgemm(const double *A, int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, int64_t M, int64_t N,
              int64_t K, void *stream);
 
+/* A batch of products C_b = A_b * B_b (b < batch) with element strides sA,
+ * sB, sC between consecutive products (0 broadcasts an operand): the batched
+ * contraction that ArrayAnsatz.contract's einsum becomes once its labels are
+ * grouped into (batch, rows, contracted, columns) (array_ansatz.py:159-225,
+ * backend_manager.py:526-548).  Same summation as mm_zgemm. */
+int mm_zgemm_batched(const double *A, int64_t lda, int64_t sA, const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc,
+                     int64_t sC, int64_t M, int64_t N, int64_t K, int64_t batch, void *stream);
+
 /* ------------------------------------------------- work-queue batch (plans)
  *
  * A plan describes n triples of uneven mode count D_k <= 8 and shape_k.

@@ -7,6 +7,13 @@ mrmustard/math/backend_manager.py:526-548).  When the contracted indices can
 be grouped, that einsum is one complex matrix product; ``zgemm_hip`` runs it
 with ``mm_zgemm`` of libmmhip.so (v_mfma_f64_16x16x4_f64 tiles).
 
+``contract`` is ``ArrayAnsatz.contract`` (array_ansatz.py:159-225) for two
+arrays: labels shared by both operands and the output form a batch, labels
+shared by the operands only are summed, the rest are the rows and columns;
+one permutation per operand, one batched product (``mm_zgemm_batched``) and a
+permutation of the result.  ``einsum`` is the two-operand case of
+``math.einsum`` (backend_manager.py:526-548) on the same path.
+
 ``dm_operator_product`` is that contraction for an n-mode density matrix in
 the reference's wire order (all bra indices, then all ket indices;
 wires.py:101-106): the (bra multi-index) x (ket multi-index) matrix is
@@ -22,7 +29,7 @@ import torch
 from .. import _lib
 from ..lattice.strategies import as_device_c128
 
-__all__ = ["zgemm_hip", "dm_operator_product"]
+__all__ = ["contract", "dm_operator_product", "einsum", "zgemm_batched_hip", "zgemm_hip"]
 
 
 def zgemm_hip(A, B, out: torch.Tensor | None = None) -> torch.Tensor:
@@ -69,3 +76,107 @@ def dm_operator_product(P, rho, out: torch.Tensor | None = None) -> torch.Tensor
     P = as_device_c128(P).reshape(d, d)
     C = zgemm_hip(P, rho.reshape(d, d), out=None if out is None else out.reshape(d, d))
     return C.reshape(rho.shape)
+
+
+def zgemm_batched_hip(A, B) -> torch.Tensor:
+    """C[b] = A[b] @ B[b] for (batch, M, K) and (batch, K, N) complex128
+    tensors; an operand with batch 1 is broadcast."""
+    A = as_device_c128(A)
+    B = as_device_c128(B)
+    if A.dim() != 3 or B.dim() != 3:
+        raise ValueError(f"zgemm_batched_hip needs (batch, M, K) and (batch, K, N), got {tuple(A.shape)} and {tuple(B.shape)}")
+    ba, M, K = A.shape
+    bb, K2, N = B.shape
+    if K != K2:
+        raise ValueError(f"inner dimensions differ: {tuple(A.shape)} @ {tuple(B.shape)}")
+    if ba != bb and 1 not in (ba, bb):
+        raise ValueError(f"batch sizes differ: {ba} and {bb}")
+    nb = max(ba, bb)
+    out = torch.empty((nb, M, N), dtype=torch.complex128, device=A.device)
+    if K == 0 or out.numel() == 0:
+        return out.zero_()
+    L = _lib.load()
+    rc = L.mm_zgemm_batched(
+        ctypes.c_void_p(A.data_ptr()), K, 0 if ba == 1 else M * K, ctypes.c_void_p(B.data_ptr()), N, 0 if bb == 1 else K * N,
+        ctypes.c_void_p(out.data_ptr()), N, M * N, M, N, K, nb, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream),
+    )
+    _lib.check(rc, "mm_zgemm_batched")
+    return out
+
+
+def contract(array1, array2, idx1, idx2, idx_out) -> torch.Tensor:
+    """``ArrayAnsatz.contract`` (array_ansatz.py:159-225) of two arrays on the
+    GPU: labels (str or int) name the axes of each array; shared labels whose
+    sizes differ are cut to the smaller size, as the reference does; the result
+    has the axes ``idx_out``.  Each label may appear once per operand; a label
+    of one operand that is not in the output is summed over."""
+    a = as_device_c128(array1)
+    b = as_device_c128(array2)
+    idx1, idx2, idx_out = list(idx1), list(idx2), list(idx_out)
+    if len(idx1) != a.dim():
+        raise ValueError(f"expected len(idx1)={a.dim()} got {len(idx1)}")
+    if len(idx2) != b.dim():
+        raise ValueError(f"expected len(idx2)={b.dim()} got {len(idx2)}")
+    if not set(idx_out).issubset(set(idx1) | set(idx2)):
+        raise ValueError("Output labels must be present in input labels.")
+    for idx in (idx1, idx2, idx_out):
+        if len(set(idx)) != len(idx):
+            raise NotImplementedError("a label repeated within one operand (a trace or diagonal) is not supported")
+    shared = set(idx1) & set(idx2)
+    sl1 = [slice(None)] * a.dim()
+    sl2 = [slice(None)] * b.dim()
+    for lab in shared:
+        p1, p2 = idx1.index(lab), idx2.index(lab)
+        d = min(a.shape[p1], b.shape[p2])
+        sl1[p1] = slice(0, d)
+        sl2[p2] = slice(0, d)
+    a = a[tuple(sl1)]
+    b = b[tuple(sl2)]
+    # a label of one operand only, absent from the output, is summed first
+    for arr, idx in ((0, idx1), (1, idx2)):
+        lone = [lab for lab in idx if lab not in shared and lab not in idx_out]
+        if lone:
+            t = a if arr == 0 else b
+            t = t.sum(dim=[idx.index(lab) for lab in lone])
+            for lab in lone:
+                idx.remove(lab)
+            if arr == 0:
+                a = t
+            else:
+                b = t
+    batch = [lab for lab in idx_out if lab in shared]
+    summed = [lab for lab in idx1 if lab in shared and lab not in idx_out]
+    rows = [lab for lab in idx_out if lab in idx1 and lab not in shared]
+    cols = [lab for lab in idx_out if lab in idx2 and lab not in shared]
+
+    def size(t, idx, labs):
+        n = 1
+        for lab in labs:
+            n *= int(t.shape[idx.index(lab)])
+        return n
+
+    A3 = a.permute(*[idx1.index(lab) for lab in batch + rows + summed]).reshape(
+        size(a, idx1, batch), size(a, idx1, rows), size(a, idx1, summed))
+    B3 = b.permute(*[idx2.index(lab) for lab in batch + summed + cols]).reshape(
+        size(b, idx2, batch), size(b, idx2, summed), size(b, idx2, cols))
+    C3 = zgemm_batched_hip(A3.contiguous(), B3.contiguous())
+    dims = [int(a.shape[idx1.index(lab)]) for lab in batch + rows] + [int(b.shape[idx2.index(lab)]) for lab in cols]
+    order = batch + rows + cols
+    C = C3.reshape(dims)
+    if not idx_out:
+        return C
+    return C.permute(*[order.index(lab) for lab in idx_out]).contiguous()
+
+
+def einsum(string: str, *tensors) -> torch.Tensor:
+    """``math.einsum`` (backend_manager.py:526-548) for two operands whose
+    labels each appear once per operand, e.g. ``"abc,cd->abd"``."""
+    if len(tensors) != 2:
+        raise NotImplementedError(f"einsum on the GPU takes two operands, got {len(tensors)}")
+    if "->" not in string:
+        raise NotImplementedError("einsum on the GPU needs an explicit output ('->')")
+    ins, out = string.replace(" ", "").split("->")
+    parts = ins.split(",")
+    if len(parts) != 2 or "." in string:
+        raise ValueError(f"bad einsum string {string!r} for two operands")
+    return contract(tensors[0], tensors[1], list(parts[0]), list(parts[1]), list(out))

@@ -31,11 +31,15 @@ struct ZgemmArgs {
     const double2 *A, *B;
     double2 *C;
     int64_t M, N, K, lda, ldb, ldc;
+    int64_t sA, sB, sC;  // element strides between the products of a batch (grid z; 0 broadcasts an operand)
 };
 
 constexpr int ZG_BM = 64, ZG_BN = 128, ZG_BK = 8, ZG_THREADS = 256;
 
 __global__ __launch_bounds__(ZG_THREADS, 2) void zgemm_mfma_kernel(ZgemmArgs a) {
+    a.A += int64_t(blockIdx.z) * a.sA;
+    a.B += int64_t(blockIdx.z) * a.sB;
+    a.C += int64_t(blockIdx.z) * a.sC;
     // LDS planes, two buffers: A transposed [k][m] (rows padded by one double
     // so that the transposing stores of consecutive lanes fall in different
     // banks), B [k][n]

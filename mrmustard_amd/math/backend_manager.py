@@ -173,6 +173,15 @@ class BackendManager:
         (backend_manager.py:759-786)."""
         return self._apply("hermite_renormalized_1leftoverMode", (A, b, c, output_cutoff, pnr_cutoffs, reorderedAB))
 
+    def einsum(self, string: str, *tensors, optimize="greedy", backend=None):
+        r"""The Einstein summation of two Fock-basis arrays on the FP64 matrix
+        cores: the contraction ``ArrayAnsatz.contract`` hands to ``math.einsum``
+        (backend_manager.py:526-548).  ``optimize`` and ``backend`` are accepted
+        for the reference's signature; a pair has one contraction order."""
+        from ..contraction import einsum as _einsum
+
+        return _einsum(string, *tensors)
+
     def _as_input(self, x):
         if hasattr(x, "ndim") and hasattr(x, "shape"):
             return x

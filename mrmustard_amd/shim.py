@@ -47,23 +47,6 @@ class HipBackendWrapper:
         G = st.vanilla_batch_hip(tuple(shape), A, b, c, stable=stable)
         return self._to_numpy(G, out)
 
-
-def install(manager) -> HipBackendWrapper:
-    """Wrap ``manager``'s current backend (instance attribute, so the class
-    default is untouched and ``uninstall`` restores it)."""
-    current = manager._backend
-    if isinstance(current, HipBackendWrapper):
-        return current
-    wrapper = HipBackendWrapper(current)
-    manager._backend = wrapper
-    return wrapper
-
-
-def uninstall(manager) -> None:
-    current = manager.__dict__.get("_backend")
-    if isinstance(current, HipBackendWrapper):
-        manager._backend = current._base
-
     def hermite_renormalized_binomial(self, A, B, C, shape, max_l2, global_cutoff):
         from .lattice import strategies as st
         from .settings import settings
@@ -87,3 +70,20 @@ def uninstall(manager) -> None:
         from .lattice import compact_fock as cf
 
         return self._to_numpy(cf.leftover_hip(A, b, c, output_cutoff, pnr_cutoffs, reordered=True), None)
+
+
+def install(manager) -> HipBackendWrapper:
+    """Wrap ``manager``'s current backend (instance attribute, so the class
+    default is untouched and ``uninstall`` restores it)."""
+    current = manager._backend
+    if isinstance(current, HipBackendWrapper):
+        return current
+    wrapper = HipBackendWrapper(current)
+    manager._backend = wrapper
+    return wrapper
+
+
+def uninstall(manager) -> None:
+    current = manager.__dict__.get("_backend")
+    if isinstance(current, HipBackendWrapper):
+        manager._backend = current._base

@@ -95,3 +95,7 @@ def test_shim_wiring():
     assert shim.install(m) is w
     shim.uninstall(m)
     assert isinstance(m._backend, FakeNumpyBackend)
+    # the numpy backend's Hermite methods (backend_numpy.py:381-446) are the wrapper's own
+    for name in ("hermite_renormalized", "hermite_renormalized_batched", "hermite_renormalized_binomial",
+                 "hermite_renormalized_diagonal", "hermite_renormalized_1leftoverMode"):
+        assert name in vars(shim.HipBackendWrapper), name
