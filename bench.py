@@ -272,7 +272,24 @@ def main():
                 "rel_err_first_64_rows_vs_torch_matmul": c_err,
             },
         }
-        del G4, Cp, P, V
+        del Cp, P, V
+        # a two-mode operator on modes 0, 1 of the same density matrix (ArrayAnsatz.contract labels):
+        # bra side (no permutation: one 100 x 10^6 x 100 product) and ket side (operand and result permuted)
+        log("4-mode DM (10,)^8: two-mode operator, bra and ket side (contract)")
+        from mrmustard_amd.contraction import contract
+
+        U2 = torch.randn(10, 10, 10, 10, dtype=torch.complex128, device=dev)
+        t_bra = timed(lambda: contract(U2, G4, ["A", "B", 0, 1], [0, 1, 2, 3, 4, 5, 6, 7], ["A", "B", 2, 3, 4, 5, 6, 7]), 3, 1)
+        t_ket = timed(lambda: contract(G4, U2, [0, 1, 2, 3, 4, 5, 6, 7], [4, 5, "A", "B"], [0, 1, 2, 3, "A", "B", 6, 7]), 3, 1)
+        gflop = 8.0 * 100 * 10**6 * 100 / 1e9
+        extra["dm8_10x8"]["two_mode_operator"] = {
+            "bra_side_ms": t_bra * 1e3,
+            "bra_side_tflops": gflop / t_bra / 1e3,
+            "ket_side_ms": t_ket * 1e3,
+            "ket_side_tflops": gflop / t_ket / 1e3,
+            "note": "8e10 flop per product; the ket side adds a permutation of rho and of the result (2 x 3.2 GB)",
+        }
+        del G4, U2
         # ---- binomial strategy (fill by weight, L2 stop) on one (30,30,30) ket
         log("binomial: one (30,30,30) ket, max_l2 = 0.99999")
         A0, b0, c0 = A[0], b[0], c[0]
