@@ -15,9 +15,10 @@
 // location it read it from (its own inbox slot, or the d >= 1 sub-lattice
 // entry), and the producer reads the adjoints of its outputs back at the
 // locations it wrote them to: the adjoint arrays mirror the forward's.  Lane
-// i owns row i of dA and dB_i; each block keeps its sums in registers over a
-// grid-stride loop and adds them to its own partial row (the same block slot
-// in every launch), summed in block order at the end: deterministic.
+// i owns row i of dA and dB_i, kept in its own LDS column over a grid-stride
+// loop; each block adds them to its own partial row (the same block slot in
+// every launch), and the rows are summed in a fixed order at the end:
+// deterministic.
 #pragma once
 
 #include "diagonal.hpp"
@@ -42,6 +43,7 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
     __shared__ double2 sx[G][PB + 1];
     __shared__ double2 sw[G][PB + 1];
     __shared__ double2 sg[PB];
+    __shared__ double2 sacc[G + 1][DG_THREADS];  // lane t's dA row (l < G) and dB (l == G), column t
     const DiagArgs &a = v.a;
     const int t = threadIdx.x;
     for (int e = t; e < G * G; e += DG_THREADS) sA[e / G][e % G] = a.A[src_index<M, REORDER>(e / G) * G + src_index<M, REORDER>(e % G)];
@@ -49,10 +51,8 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
     const int slot = t % PB, i = t / PB, m = i >> 1;
     const bool lane_ok = i < G;
     const int64_t P = a.P;
-    double2 acc[G];
 #pragma unroll
-    for (int l = 0; l < G; ++l) acc[l] = make_double2(0.0, 0.0);
-    double2 accB = make_double2(0.0, 0.0);
+    for (int l = 0; l <= G; ++l) sacc[l][t] = make_double2(0.0, 0.0);
     __syncthreads();
 
     for (int it = 0; it < iters; ++it) {
@@ -140,8 +140,8 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
             __syncthreads();
             if (run) {
 #pragma unroll
-                for (int l = 2 * d; l < G; ++l) acc[l] = cfma(wv, sx[l][slot], acc[l]);
-                accB = cfma(wv, sg[slot], accB);
+                for (int l = 2 * d; l < G; ++l) sacc[l][t] = cfma(wv, sx[l][slot], sacc[l][t]);
+                sacc[G][t] = cfma(wv, sg[slot], sacc[G][t]);
                 // lane i as column l = i: sum_i' w_i' A_i'l, times K_l[l]
                 double2 s = make_double2(0.0, 0.0);
 #pragma unroll
@@ -198,8 +198,8 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
         double2 gdiag = make_double2(0.0, 0.0);
         if (diag) {
 #pragma unroll
-            for (int l = 0; l < G; ++l) acc[l] = cfma(wv, sx[l][slot], acc[l]);
-            accB = cfma(wv, g0, accB);
+            for (int l = 0; l < G; ++l) sacc[l][t] = cfma(wv, sx[l][slot], sacc[l][t]);
+            sacc[G][t] = cfma(wv, g0, sacc[G][t]);
             double2 s = make_double2(0.0, 0.0);
 #pragma unroll
             for (int r = 0; r < G; ++r) s = cfma(sw[r][slot], sA[r][i], s);
@@ -226,31 +226,35 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
 
     // per-block sums over the PB params slots of each row, added to this block's partial row
     double2 *prow = v.part + int64_t(blockIdx.x) * G * (G + 1);
-#pragma unroll
-    for (int l = 0; l <= G; ++l) {
-        if (lane_ok) sx[i][slot] = l < G ? acc[l] : accB;
-        __syncthreads();
-        if (t < G) {
-            double2 s = prow[t * (G + 1) + l];
-            for (int k = 0; k < PB; ++k) s = cadd(s, sx[t][k]);
-            prow[t * (G + 1) + l] = s;
-        }
-        __syncthreads();
+    for (int e = t; e < G * (G + 1); e += DG_THREADS) {
+        const int r = e / (G + 1), l = e % (G + 1);
+        double2 s = prow[e];
+        for (int k = 0; k < PB; ++k) s = cadd(s, sacc[l][r * PB + k]);
+        prow[e] = s;
     }
 }
 
-// dA (caller's order), dB: partial rows summed in block order.
+// dA (caller's order), dB: one block per entry; thread k sums partial rows
+// k, k + 256, ... in order, then a fixed tree over the 256 threads.
 template <int M, bool REORDER>
 __global__ __launch_bounds__(256) void diag_vjp_finalize(const double2 *part, int nblk, double2 *dA, double2 *dB) {
     constexpr int G = 2 * M;
-    for (int e = threadIdx.x; e < G * (G + 1); e += blockDim.x) {
-        double2 s = make_double2(0.0, 0.0);
-        for (int b = 0; b < nblk; ++b) s = cadd(s, part[int64_t(b) * G * (G + 1) + e]);
+    __shared__ double2 red[256];
+    const int e = blockIdx.x, k = threadIdx.x;
+    double2 s = make_double2(0.0, 0.0);
+    for (int b = k; b < nblk; b += 256) s = cadd(s, part[int64_t(b) * G * (G + 1) + e]);
+    red[k] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (k < h) red[k] = cadd(red[k], red[k + h]);
+        __syncthreads();
+    }
+    if (k == 0) {
         const int r = e / (G + 1), l = e % (G + 1);
         if (l < G)
-            dA[src_index<M, REORDER>(r) * G + src_index<M, REORDER>(l)] = s;
+            dA[src_index<M, REORDER>(r) * G + src_index<M, REORDER>(l)] = red[0];
         else
-            dB[src_index<M, REORDER>(r)] = s;
+            dB[src_index<M, REORDER>(r)] = red[0];
     }
 }
 

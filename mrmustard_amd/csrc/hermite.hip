@@ -858,12 +858,14 @@ void launch_diag_vjp(DiagVjpArgs &v, const std::vector<int64_t> &counts, int nbl
     hipLaunchKernelGGL(zero_c128_span_kernel, dim3(64), dim3(256), 0, s, v.part, np);
     std::vector<int64_t> starts(counts.size(), 0);
     for (size_t w = 1; w < counts.size(); ++w) starts[w] = starts[w - 1] + counts[w - 1];
-    const int64_t per = int64_t(nblk) * PB;
     for (int w = v.a.S; w >= 0; --w) {
+        // blocks 0 .. nw-1 of the fixed nblk slots: block b always adds to partial row b
+        const int nw = int(std::min<int64_t>(nblk, (counts[w] + PB - 1) / PB));
+        const int64_t per = int64_t(nw) * PB;
         const int iters = int((counts[w] + per - 1) / per);
-        hipLaunchKernelGGL((diag_vjp_kernel<M, REORDER>), dim3(nblk), dim3(DG_THREADS), 0, s, v, w, counts[w], starts[w], iters);
+        hipLaunchKernelGGL((diag_vjp_kernel<M, REORDER>), dim3(nw), dim3(DG_THREADS), 0, s, v, w, counts[w], starts[w], iters);
     }
-    hipLaunchKernelGGL((diag_vjp_finalize<M, REORDER>), dim3(1), dim3(256), 0, s, v.part, nblk, dA, dB);
+    hipLaunchKernelGGL((diag_vjp_finalize<M, REORDER>), dim3(2 * M * (2 * M + 1)), dim3(256), 0, s, v.part, nblk, dA, dB);
 }
 
 // ------------------------------------------------------------ one leftover mode
