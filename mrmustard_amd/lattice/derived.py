@@ -1,0 +1,94 @@
+"""Fock arrays of Bargmann triples with derived variables, on the GPU.
+
+A ``PolyExpAnsatz`` with n continuous and m derived variables has A, b over
+n + m variables (batch first) and c of shape ``batch + shape_derived_vars``.
+
+- ``fock_array_derived`` <- CircuitComponent.fock_array's ``num_derived_vars > 0``
+  branch (mrmustard/lab/circuit_components.py:516-528): the lattice on
+  ``shape + shape_derived_vars`` with c = 1, its derived indices flattened and
+  contracted with c.
+- ``decompose_ansatz_triple`` <- PolyExpAnsatz.decompose_ansatz
+  (mrmustard/physics/ansatz/polyexp_ansatz.py:415-469): the equivalent triple
+  with n derived variables of size sum(shape_derived_vars) each, for m >= n.
+
+Both run the batched Hermite kernels and one batched product on the FP64
+matrix cores (``zgemm_batched_hip``) for the contraction with c.
+"""
+
+from __future__ import annotations
+
+import math as pymath
+
+import torch
+
+from ..contraction import zgemm_batched_hip
+from .strategies import as_device_c128
+
+__all__ = ["decompose_ansatz_triple", "fock_array_derived"]
+
+
+def _split(A, b, c):
+    A, b, c = as_device_c128(A), as_device_c128(b), as_device_c128(c)
+    batch_shape = tuple(A.shape[:-2])
+    if tuple(b.shape[:-1]) != batch_shape or tuple(c.shape[: len(batch_shape)]) != batch_shape:
+        raise ValueError(f"batch shapes of A {tuple(A.shape)}, b {tuple(b.shape)} and c {tuple(c.shape)} differ")
+    shape_derived = tuple(int(k) for k in c.shape[len(batch_shape):])
+    n = int(A.shape[-1]) - len(shape_derived)
+    if n < 0:
+        raise ValueError("c has more derived axes than A has variables")
+    return A, b, c, batch_shape, shape_derived, n
+
+
+def _lattice(A, b, batch_shape, shape):
+    from .. import math as mm
+
+    ones = torch.ones(batch_shape, dtype=torch.complex128, device=A.device)
+    return mm.hermite_renormalized(A, b, ones, shape=tuple(shape))
+
+
+def _contract_derived(G, c, batch_shape, head_shape):
+    """sum_k G[..., head, k] c[..., k] with the derived axes flattened into k."""
+    nb = pymath.prod(batch_shape) if batch_shape else 1
+    rows = pymath.prod(head_shape) if head_shape else 1
+    G3 = G.reshape(nb, rows, -1)
+    c3 = c.reshape(nb, -1, 1)
+    return zgemm_batched_hip(G3, c3).reshape(tuple(batch_shape) + tuple(head_shape))
+
+
+def fock_array_derived(A, b, c, shape) -> torch.Tensor:
+    """The Fock array of shape ``batch + shape`` of a triple whose c carries the
+    derived variables (circuit_components.py:516-528): the lattice of (A, b, 1)
+    on ``shape + shape_derived_vars``, contracted over the derived indices
+    with c.  ``shape`` has one entry per continuous variable."""
+    A, b, c, batch_shape, shape_derived, n = _split(A, b, c)
+    shape = tuple(int(s) for s in shape)
+    if len(shape) != n:
+        raise ValueError(f"shape {shape} has {len(shape)} entries for {n} continuous variables")
+    G = _lattice(A, b, batch_shape, shape + shape_derived)
+    return _contract_derived(G, c, batch_shape, shape)
+
+
+def decompose_ansatz_triple(A, b, c):
+    """(A', b', c') of PolyExpAnsatz.decompose_ansatz (polyexp_ansatz.py:415-469):
+    for m derived variables with m >= n continuous ones,
+      A' = [[A_nn, I], [I, 0]],  b' = [b_n, 0],
+      c'[..., l_1..l_n] = sum_k H[..., l, k] c[..., k],  l_i < sum(shape_derived_vars),
+    with H the lattice of ([[0, A_nm], [A_mn, A_mm]], [0, b_m], 1).  Returns
+    the triple unchanged when m < n."""
+    A, b, c, batch_shape, shape_derived, n = _split(A, b, c)
+    m = len(shape_derived)
+    if m < n:
+        return A, b, c
+    L = sum(shape_derived)
+    pulled = (L,) * n
+    A_core = A.clone()
+    A_core[..., :n, :n] = 0
+    b_core = b.clone()
+    b_core[..., :n] = 0
+    H = _lattice(A_core, b_core, batch_shape, pulled + shape_derived)
+    c_prime = _contract_derived(H, c, batch_shape, pulled)
+    eye = torch.eye(n, dtype=torch.complex128, device=A.device).expand(*batch_shape, n, n)
+    zero = torch.zeros(*batch_shape, n, n, dtype=torch.complex128, device=A.device)
+    A_dec = torch.cat([torch.cat([A[..., :n, :n], eye], dim=-1), torch.cat([eye, zero], dim=-1)], dim=-2).contiguous()
+    b_dec = torch.cat([b[..., :n], torch.zeros(*batch_shape, n, dtype=torch.complex128, device=A.device)], dim=-1).contiguous()
+    return A_dec, b_dec, c_prime
