@@ -96,6 +96,11 @@ def main():
         action="store_true",
         help="only run a plain 1 GiB device-to-device copy (calibrates the HBM byte counters)",
     )
+    ap.add_argument(
+        "--diagonal-probe",
+        action="store_true",
+        help="only run the diagonal strategy on the 6-mode density matrix, cutoffs 10, three times (for counters)",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
@@ -116,6 +121,16 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}))
+        return
+    if args.diagonal_probe:
+        from mrmustard_amd.lattice.compact_fock import diagonal_hip
+
+        Ad, bd, cd = W.pnr_dm(6)
+        Adt, bdt, cdt = (torch.as_tensor(np.asarray(x), device=dev) for x in (Ad, bd, cd))
+        for _ in range(3):
+            diagonal_hip(Adt, bdt, cdt, (10,) * 6, reordered=True)
+        torch.cuda.synchronize()
+        print(json.dumps({"diagonal_probe_values_written": diag_values_written((10,) * 6), "calls": 3}))
         return
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
 
