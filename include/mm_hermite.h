@@ -132,9 +132,12 @@ int mm_hermite_binomial(const double *A, const double *b, const double *c, const
  * (mrmustard/math/lattice/autoshape.py:23-154), called by State.auto_shape
  * (mrmustard/lab/states/base.py:427-434).  A: B x (2M x 2M), indices
  * (bra modes, ket modes); b: B x 2M; c: B; shape: B x M int64 (device).  A
- * mode whose complement matrix is singular gets -1 (the reference raises). */
+ * mode whose complement matrix is singular gets -1 (the reference raises).
+ * The workspace holds each (state, mode)'s single-mode marginal between the
+ * two launches (mm_autoshape_workspace bytes). */
+size_t mm_autoshape_workspace(int64_t B, int M);
 int mm_autoshape(const double *A, const double *b, const double *c, int64_t B, int M, double max_prob, int max_shape,
-                 int min_shape, int64_t *shape, void *stream);
+                 int min_shape, int64_t *shape, void *workspace, size_t ws_bytes, void *stream);
 
 /* ----------------------------------------------------------------- diagonal
  *

@@ -48,7 +48,8 @@ SIGNATURES = {
     "mm_hermite_binomial": (_int, [_vp, _vp, _vp, _vp, _int, ctypes.c_double, _i64, _vp, _vp, _vp, _sz, _vp]),
     "mm_zgemm": (_int, [_vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
     "mm_zgemm_batched": (_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp]),
-    "mm_autoshape": (_int, [_vp, _vp, _vp, _i64, _int, ctypes.c_double, _int, _int, _vp, _vp]),
+    "mm_autoshape_workspace": (_sz, [_i64, _int]),
+    "mm_autoshape": (_int, [_vp, _vp, _vp, _i64, _int, ctypes.c_double, _int, _int, _vp, _vp, _sz, _vp]),
     "mm_hermite_diagonal_workspace": (_sz, [_vp, _int, _i64]),
     "mm_hermite_diagonal": (_int, [_vp, _vp, _vp, _int, _vp, _int, _i64, _int, _vp, _vp, _sz, _vp]),
     "mm_hermite_diagonal_vjp_workspace": (_sz, [_vp, _int]),

@@ -1,15 +1,16 @@
 // Fock cutoffs of Gaussian density matrices: autoshape_numba
 // (mrmustard/math/lattice/autoshape.py:23-154) for a batch of states.
 //
-// One wave per (state, mode m).  The reduced state of mode m is the Schur
-// complement (autoshape.py:84-108):
+// Two launches.  One wave per (state, mode m) forms the reduced state of mode
+// m, the Schur complement (autoshape.py:84-108):
 //   R   = A_nn - X,  X = [[0, I], [I, 0]] over the other M-1 modes,
 //   A_  = A_mm - A_mn R^-1 A_nm,  b_ = b_m - A_mn R^-1 b_n,
 //   c_  = c exp(-b_n R^-1 b_n / 2) / sqrt(det R).
 // The wave solves R Y = [A_nm | b_n] by Gauss-Jordan elimination with partial
-// pivoting in LDS (det R from the pivots), then lane 0 runs the reference's
-// early-stopped diagonal recursion (autoshape.py:110-153) and writes the
-// step count clipped to [min_shape, max_shape] (autoshape.py:154).
+// pivoting in LDS (det R from the pivots).  Then one lane per (state, mode)
+// runs the reference's early-stopped diagonal recursion (autoshape.py:110-153),
+// a dependent chain of up to max_shape steps, and writes the step count
+// clipped to [min_shape, max_shape] (autoshape.py:154).
 //
 // Index layout of the 2M-dimensional A and b: entry s*M + i is (bra if s = 0,
 // ket if s = 1) of mode i, as the reference's reshape (2, M, 2, M) reads it.
@@ -30,6 +31,7 @@ struct AutoshapeArgs {
     int max_shape, min_shape;
     const double *sq;  // sqrt(n), n < 100000
     int64_t *shape;    // B x M
+    double2 *marg;     // B x M x 8: the single-mode (A_00, A_01, A_10, A_11, b_0, b_1, c_) and a flag
 };
 
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -51,7 +53,8 @@ __device__ __forceinline__ double2 cexp_(double2 z) {
     return make_double2(e * cos(z.y), e * sin(z.y));
 }
 
-__global__ __launch_bounds__(64) void autoshape_kernel(AutoshapeArgs a) {
+// Wave per (state, mode): the Schur complement, written to marg for the recursion.
+__global__ __launch_bounds__(64) void autoshape_schur_kernel(AutoshapeArgs a) {
     __shared__ double2 W[AS_MAXN][AS_MAXN + 3];
     __shared__ double2 Amn[2][AS_MAXN];
     __shared__ int piv_row;
@@ -140,8 +143,35 @@ __global__ __launch_bounds__(64) void autoshape_kernel(AutoshapeArgs a) {
     for (int r = 0; r < n2; ++r) qf = cadd(qf, cmul(b[ridx(r)], W[r][n2 + 2]));
     const double2 ex = cexp_(make_double2(-0.5 * qf.x, -0.5 * qf.y));
     const double2 cm = cdiv_py(cmul(a.c[st], ex), csqrt_principal(det));
+    double2 *o = a.marg + (st * M + m) * 8;
+    o[0] = Am[0][0];
+    o[1] = Am[0][1];
+    o[2] = Am[1][0];
+    o[3] = Am[1][1];
+    o[4] = bm[0];
+    o[5] = bm[1];
+    o[6] = cm;
+    // a singular R (np.linalg.inv raises there) is reported as -1
+    o[7] = make_double2((isfinite(cm.x) && isfinite(cm.y)) ? 0.0 : 1.0, 0.0);
+}
 
-    // diagonal recursion (buffers transposed with respect to the reference's diagram)
+// Lane per (state, mode): the reference's early-stopped diagonal recursion
+// (autoshape.py:110-153) on the single-mode marginal, and the step count
+// clipped to [min_shape, max_shape] (autoshape.py:154).
+__global__ __launch_bounds__(256) void autoshape_recursion_kernel(AutoshapeArgs a) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= a.B * a.M) return;
+    const double2 *o = a.marg + q * 8;
+    double2 Am[2][2], bm[2];
+    Am[0][0] = o[0];
+    Am[0][1] = o[1];
+    Am[1][0] = o[2];
+    Am[1][1] = o[3];
+    bm[0] = o[4];
+    bm[1] = o[5];
+    const double2 cm = o[6];
+    const bool bad = o[7].x != 0.0;
+    // buffers transposed with respect to the reference's diagram
     const double *SQ = a.sq;
     double2 buf2[2][2], buf3[2][3];
     for (int i = 0; i < 2; ++i) {
@@ -151,20 +181,20 @@ __global__ __launch_bounds__(64) void autoshape_kernel(AutoshapeArgs a) {
     buf3[0][1] = cm;
     double norm = hypot(cm.x, cm.y);
     int k = 0;
-    while (norm < a.max_prob && k < a.max_shape) {
-        const int o = k & 1, n = o ^ 1;
+    while (!bad && norm < a.max_prob && k < a.max_shape) {
+        const int o2 = k & 1, n = o2 ^ 1;
         const double s0 = SQ[k], s1 = SQ[k + 1], s2 = SQ[k + 2];
         for (int u = 0; u < 2; ++u) {
-            const double2 av = cadd(cmul(Am[u][0], buf2[o][0]), cmul(Am[u][1], buf2[o][1]));
-            const double2 t = cadd(cmul(bm[u], buf3[o][1]), make_double2(av.x * s0, av.y * s0));
+            const double2 av = cadd(cmul(Am[u][0], buf2[o2][0]), cmul(Am[u][1], buf2[o2][1]));
+            const double2 t = cadd(cmul(bm[u], buf3[o2][1]), make_double2(av.x * s0, av.y * s0));
             buf2[n][u] = make_double2(t.x / s1, t.y / s1);
         }
-        double2 t0 = cadd(cadd(cmul(bm[0], buf2[n][0]), cmul(make_double2(Am[0][0].x * s1, Am[0][0].y * s1), buf3[o][1])),
-                          cmul(make_double2(Am[0][1].x * s0, Am[0][1].y * s0), buf3[o][0]));
-        double2 t1 = cadd(cadd(cmul(bm[1], buf2[n][0]), cmul(make_double2(Am[1][0].x * s1, Am[1][0].y * s1), buf3[o][1])),
-                          cmul(make_double2(Am[1][1].x * s0, Am[1][1].y * s0), buf3[o][0]));
-        double2 t2 = cadd(cadd(cmul(bm[1], buf2[n][1]), cmul(make_double2(Am[1][0].x * s0, Am[1][0].y * s0), buf3[o][2])),
-                          cmul(make_double2(Am[1][1].x * s1, Am[1][1].y * s1), buf3[o][1]));
+        double2 t0 = cadd(cadd(cmul(bm[0], buf2[n][0]), cmul(make_double2(Am[0][0].x * s1, Am[0][0].y * s1), buf3[o2][1])),
+                          cmul(make_double2(Am[0][1].x * s0, Am[0][1].y * s0), buf3[o2][0]));
+        double2 t1 = cadd(cadd(cmul(bm[1], buf2[n][0]), cmul(make_double2(Am[1][0].x * s1, Am[1][0].y * s1), buf3[o2][1])),
+                          cmul(make_double2(Am[1][1].x * s0, Am[1][1].y * s0), buf3[o2][0]));
+        double2 t2 = cadd(cadd(cmul(bm[1], buf2[n][1]), cmul(make_double2(Am[1][0].x * s0, Am[1][0].y * s0), buf3[o2][2])),
+                          cmul(make_double2(Am[1][1].x * s1, Am[1][1].y * s1), buf3[o2][1]));
         buf3[n][0] = make_double2(t0.x / s2, t0.y / s2);
         buf3[n][1] = make_double2(t1.x / s1, t1.y / s1);
         buf3[n][2] = make_double2(t2.x / s2, t2.y / s2);
@@ -172,9 +202,7 @@ __global__ __launch_bounds__(64) void autoshape_kernel(AutoshapeArgs a) {
         ++k;
     }
     const int kc = k < a.min_shape ? a.min_shape : (k > a.max_shape ? a.max_shape : k);
-    // a singular R (np.linalg.inv raises there) is reported as -1
-    const bool bad = !(isfinite(cm.x) && isfinite(cm.y));
-    a.shape[st * M + m] = bad ? -1 : kc;
+    a.shape[q] = bad ? -1 : kc;
 }
 
 }  // namespace mm

@@ -4,7 +4,7 @@
 min_shape)`` (mrmustard/math/lattice/autoshape.py:23-154): A is the 2M x 2M
 Bargmann matrix of a density matrix with indices (bra modes, ket modes), b
 its 2M vector, c the vacuum amplitude; the result is an int64 array of M
-cutoffs.  ``autoshape_batch_hip`` does B states of the same M in one launch.
+cutoffs.  ``autoshape_batch_hip`` does B states of the same M in one call.
 Both call ``mm_autoshape`` of libmmhip.so and copy the M (or B x M) integers
 back to the host, as the caller needs them to size the Fock array.
 """
@@ -34,10 +34,12 @@ def autoshape_batch_hip(A, b, c, max_prob, max_shape, min_shape) -> np.ndarray:
         raise ValueError(f"A must be ({B}, {2 * M}, {2 * M}) and c ({B},)")
     out = torch.empty((B, M), dtype=torch.int64, device=A.device)
     L = _lib.load()
+    nbytes = int(L.mm_autoshape_workspace(B, M))
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=A.device)
     rc = L.mm_autoshape(
         ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(c.data_ptr()), B, M,
-        float(max_prob), int(max_shape), int(min_shape), ctypes.c_void_p(out.data_ptr()),
-        ctypes.c_void_p(torch.cuda.current_stream().cuda_stream),
+        float(max_prob), int(max_shape), int(min_shape), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+        nbytes, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream),
     )
     _lib.check(rc, "mm_autoshape")
     res = out.cpu().numpy()
