@@ -44,6 +44,7 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
     __shared__ double2 sw[G][PB + 1];
     __shared__ double2 sg[PB];
     __shared__ double2 sacc[G + 1][DG_THREADS];  // lane t's dA row (l < G) and dB (l == G), column t
+    __shared__ int need;                          // off-diagonal pivots some params of the block runs
     const DiagArgs &a = v.a;
     const int t = threadIdx.x;
     for (int e = t; e < G * G; e += DG_THREADS) sA[e / G][e % G] = a.A[src_index<M, REORDER>(e / G) * G + src_index<M, REORDER>(e % G)];
@@ -96,43 +97,69 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
         double2 gbar0 = make_double2(0.0, 0.0);  // this lane's share of arr0[p]'s adjoint
         double2 gext = make_double2(0.0, 0.0);   // adjoint of arr1[i, p] from the off-diagonal pivot i / 2
 
+        if (t == 0) need = 0;
+        __syncthreads();
+        if (runmask && i == 0) atomicOr(&need, runmask);
+        // every load of every pivot depends on p only: issue them all first
+        double2 xs[M], wvs[M];
+        double2 gd = make_double2(0.0, 0.0);  // arr1[2d, p] for lane 2d, sent to p + e_d's slot 2d + 1
+#pragma unroll
+        for (int d = 0; d < M; ++d) {
+            xs[d] = make_double2(0.0, 0.0);
+            wvs[d] = make_double2(0.0, 0.0);
+            if (!((runmask >> d & 1) && i >= 2 * d)) continue;
+            const int pd = p[d];
+            if (i == 2 * d) {
+                xs[d] = cscale_promoted(g0, a.sq[pd + 1]);
+                gd = a.inD[(2 * d + 1) * P + fw];
+                if (pd + 2 < a.cut[d]) {
+                    const double2 ob = d == 0 ? v.adO[P + fw] : v.ad2[a.off2[d] + f];
+                    wvs[d] = cdiv_by(ob, a.sq[pd + 2], a.rsq[pd + 2]);
+                }
+            } else if (i == 2 * d + 1) {
+                if (pd > 0) xs[d] = cscale_promoted(d == 0 ? a.inO[P + gs] : a.arr2[a.off2[d] + f - sm], a.sq[pd]);
+                wvs[d] = cdiv_by(v.adO[fw], a.sq[pd + 1], a.rsq[pd + 1]);  // arr0[p + e_d]: slot 0 of its consumer
+            } else {
+                const int64_t sl = a.off11[d] + (m - d - 1) * a.len[d];
+                if (pm > 0) {
+                    double2 xr;
+                    if (d == 0)
+                        xr = a.inO[i * P + gs];
+                    else
+                        xr = (i & 1) ? a.a1010[sl + f - sm] : a.a1001[sl + f - sm];
+                    xs[d] = cscale_promoted(xr, a.sq[pm]);
+                }
+                if (pm + 1 < cm) {
+                    double2 ob;
+                    if (d == 0)
+                        ob = v.adO[(i ^ 1) * P + fw];
+                    else
+                        ob = (i & 1) ? v.ad1001[sl + f] : v.ad1010[sl + f];
+                    wvs[d] = cdiv_by(ob, a.sq[pm + 1], a.rsq[pm + 1]);
+                }
+            }
+        }
+        // the diagonal pivot's own inputs and output adjoint (without the off-diagonal pivots' share)
+        const bool diag = act && (a.cut[0] == 1 || p[0] < a.cut[0] - 1);
+        const bool dwrite = diag && pm + 1 < cm && (i != 1 || p[0] + 2 < a.cut[0]);
+        double2 xD = make_double2(0.0, 0.0), obD = make_double2(0.0, 0.0);
+        if (diag && pm > 0) xD = cscale_promoted(a.inD[i * P + gs], a.sq[pm]);
+        if (dwrite) {
+            // the consumer p + e_m read it iff its own diagonal pivot ran
+            const int c0p = p[0] + (m == 0 ? 1 : 0);
+            if (a.cut[0] == 1 || c0p < a.cut[0] - 1) obD = v.adD[(i ^ 1) * P + fw];
+        }
+
+        __syncthreads();
+        const int need_all = need;
+
         // ---- off-diagonal pivots (diagonal_amps.py:19-105 in reverse)
 #pragma unroll
         for (int d = 0; d < M; ++d) {
+            if (!(need_all >> d & 1)) continue;  // block-uniform: no params of the block runs pivot d
             const bool run = (runmask >> d & 1) && i >= 2 * d;
-            double2 x = make_double2(0.0, 0.0), wv = make_double2(0.0, 0.0);
-            if (run) {
-                const int pd = p[d];
-                if (i == 2 * d) {
-                    x = cscale_promoted(g0, a.sq[pd + 1]);
-                    sg[slot] = a.inD[(2 * d + 1) * P + fw];  // arr1[2d, p], sent to p + e_d's slot 2d + 1
-                    if (pd + 2 < a.cut[d]) {
-                        const double2 ob = d == 0 ? v.adO[P + fw] : v.ad2[a.off2[d] + f];
-                        wv = cdiv_by(ob, a.sq[pd + 2], a.rsq[pd + 2]);
-                    }
-                } else if (i == 2 * d + 1) {
-                    if (pd > 0) x = cscale_promoted(d == 0 ? a.inO[P + gs] : a.arr2[a.off2[d] + f - sm], a.sq[pd]);
-                    wv = cdiv_by(v.adO[fw], a.sq[pd + 1], a.rsq[pd + 1]);  // arr0[p + e_d]: slot 0 of its consumer
-                } else {
-                    const int64_t sl = a.off11[d] + (m - d - 1) * a.len[d];
-                    if (pm > 0) {
-                        double2 xr;
-                        if (d == 0)
-                            xr = a.inO[i * P + gs];
-                        else
-                            xr = (i & 1) ? a.a1010[sl + f - sm] : a.a1001[sl + f - sm];
-                        x = cscale_promoted(xr, a.sq[pm]);
-                    }
-                    if (pm + 1 < cm) {
-                        double2 ob;
-                        if (d == 0)
-                            ob = v.adO[(i ^ 1) * P + fw];
-                        else
-                            ob = (i & 1) ? v.ad1001[sl + f] : v.ad1010[sl + f];
-                        wv = cdiv_by(ob, a.sq[pm + 1], a.rsq[pm + 1]);
-                    }
-                }
-            }
+            const double2 x = xs[d], wv = wvs[d];
+            if (run && i == 2 * d) sg[slot] = gd;
             if (lane_ok) {
                 sx[i][slot] = x;
                 sw[i][slot] = wv;
@@ -178,17 +205,11 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
         }
 
         // ---- diagonal pivot (diagonal_amps.py:108-150 in reverse)
-        const bool diag = act && (a.cut[0] == 1 || p[0] < a.cut[0] - 1);
-        double2 x = make_double2(0.0, 0.0), wv = make_double2(0.0, 0.0);
-        if (diag) {
-            if (pm > 0) x = cscale_promoted(a.inD[i * P + gs], a.sq[pm]);
-            if (pm + 1 < cm && (i != 1 || p[0] + 2 < a.cut[0])) {
-                // the consumer p + e_m read it iff its own diagonal pivot ran
-                const int c0p = p[0] + (m == 0 ? 1 : 0);
-                double2 ob = (a.cut[0] == 1 || c0p < a.cut[0] - 1) ? v.adD[(i ^ 1) * P + fw] : make_double2(0.0, 0.0);
-                if (!(i & 1) && (runmask >> m & 1)) ob = cadd(ob, gext);
-                wv = cdiv_by(ob, a.sq[pm + 1], a.rsq[pm + 1]);
-            }
+        double2 x = xD, wv = make_double2(0.0, 0.0);
+        if (dwrite) {
+            double2 ob = obD;
+            if (!(i & 1) && (runmask >> m & 1)) ob = cadd(ob, gext);
+            wv = cdiv_by(ob, a.sq[pm + 1], a.rsq[pm + 1]);
         }
         if (lane_ok) {
             sx[i][slot] = x;
