@@ -1366,7 +1366,9 @@ int mm_zgemm_batched(const double *A, int64_t lda, int64_t sA, const double *B, 
     if (lda < K || ldb < N || ldc < N) return fail(MM_ERR_INVALID, "leading dimension smaller than the row length");
     if (sA < 0 || sB < 0 || sC < 0) return fail(MM_ERR_INVALID, "negative batch stride");
     if (batch > 1 && sC < (M - 1) * ldc + N) return fail(MM_ERR_INVALID, "the products of a batch overlap in C");
-    const int64_t gx = (N + ZG_BN - 1) / ZG_BN, gy = (M + ZG_BM - 1) / ZG_BM;
+    const bool tall = M > 64 && M <= 128;  // one 128-row tile covers M: B is read once
+    const int bm = tall ? 128 : 64;
+    const int64_t gx = (N + ZG_BN - 1) / ZG_BN, gy = (M + bm - 1) / bm;
     if (gx > 0x7fffffff || gy > 65535) return fail(MM_ERR_UNSUPPORTED, "matrix too large for one launch");
     ZgemmArgs a;
     a.M = M;
@@ -1384,7 +1386,10 @@ int mm_zgemm_batched(const double *A, int64_t lda, int64_t sA, const double *B, 
         a.A = reinterpret_cast<const double2 *>(A) + b0 * sA;
         a.B = reinterpret_cast<const double2 *>(B) + b0 * sB;
         a.C = reinterpret_cast<double2 *>(C) + b0 * sC;
-        hipLaunchKernelGGL(zgemm_mfma_kernel, dim3(unsigned(gx), unsigned(gy), unsigned(nz)), dim3(ZG_THREADS), 0, s, a);
+        if (tall)
+            hipLaunchKernelGGL(zgemm_mfma_kernel<128>, dim3(unsigned(gx), unsigned(gy), unsigned(nz)), dim3(512), 0, s, a);
+        else
+            hipLaunchKernelGGL(zgemm_mfma_kernel<64>, dim3(unsigned(gx), unsigned(gy), unsigned(nz)), dim3(256), 0, s, a);
     }
     LAUNCH_CHECK("zgemm_mfma_kernel");
     return MM_OK;

@@ -6,8 +6,9 @@
 // backend_manager.py:526-548).  Row-major A (M x K, lda), B (K x N, ldb),
 // C (M x N, ldc), all interleaved (re, im) doubles.
 //
-// Tiling: a workgroup of 4 waves owns a 64 x 128 tile of C; wave w owns the
-// 32 x 64 sub-tile (w / 2, w % 2), i.e. 2 x 4 tiles of 16 x 16, each with a
+// Tiling: a workgroup of 4 waves owns a 64 x 128 tile of C (8 waves, 128 x 128,
+// for 64 < M <= 128); wave w owns the 32 x 64 sub-tile (w / 2, w % 2), i.e.
+// 2 x 4 tiles of 16 x 16, each with a
 // real and an imaginary accumulator (v_mfma_f64_16x16x4_f64, K = 4 per
 // instruction).  A complex product is four real ones:
 //   C_re += A_re B_re - A_im B_im,   C_im += A_re B_im + A_im B_re.
@@ -34,9 +35,14 @@ struct ZgemmArgs {
     int64_t sA, sB, sC;  // element strides between the products of a batch (grid z; 0 broadcasts an operand)
 };
 
-constexpr int ZG_BM = 64, ZG_BN = 128, ZG_BK = 8, ZG_THREADS = 256;
+constexpr int ZG_BN = 128, ZG_BK = 8;
 
-__global__ __launch_bounds__(ZG_THREADS, 2) void zgemm_mfma_kernel(ZgemmArgs a) {
+// ZG_BM = 64 (4 waves, two workgroups per CU) in general; ZG_BM = 128 (8 waves,
+// one workgroup per CU, the same two waves per SIMD) when one tile row covers
+// M (64 < M <= 128), so that B is read once instead of twice.
+template <int ZG_BM>
+__global__ __launch_bounds__(ZG_BM * 4, ZG_BM == 64 ? 2 : 1) void zgemm_mfma_kernel(ZgemmArgs a) {
+    constexpr int ZG_THREADS = ZG_BM * 4;
     a.A += int64_t(blockIdx.z) * a.sA;
     a.B += int64_t(blockIdx.z) * a.sB;
     a.C += int64_t(blockIdx.z) * a.sC;

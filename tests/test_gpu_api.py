@@ -324,7 +324,8 @@ def test_shim_backend_numbers(oracle_mod):
 # ---------------------------------------------------------------- contraction
 
 
-@pytest.mark.parametrize("mnk", [(1, 1, 1), (16, 16, 4), (64, 128, 16), (37, 150, 23), (130, 70, 200), (5, 3, 1000)],
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (16, 16, 4), (64, 128, 16), (37, 150, 23), (130, 70, 200), (5, 3, 1000),
+                                 (65, 7, 9), (100, 300, 37), (128, 129, 20)],
                          ids=lambda t: "x".join(map(str, t)))
 def test_zgemm(mnk):
     """mm_zgemm against numpy's complex matmul (the instruction sums K in
