@@ -361,3 +361,25 @@ def test_diagonal_vjp_large_vs_finite_differences():
         fd = (L(A + h * dA, b + h * db) - L(A - h * dA, b - h * db)) / (2 * h)
         an = np.sum(gA * dA) + np.sum(gB * db)
         assert abs(an - fd) < 1e-6 * max(1.0, abs(fd)), (k, an, fd)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("cutoffs", [(1,), (1, 1, 1), (2, 1), (1, 2), (1, 1, 4)], ids=lambda x: str(x))
+def test_diagonal_hip_trivial_boxes(cutoffs):
+    """Boxes with cutoffs of 1 (a single weight, or modes that never leave vacuum), forward and VJP."""
+    from oracle import diagonal as OD
+
+    from mrmustard_amd.lattice.compact_fock import diagonal_hip, diagonal_vjp_hip
+
+    M = len(cutoffs)
+    A, b, c = reference_triple(M, 90 + M)
+    Ar, br = OD.reorder_ab(A, b)
+    ref = OD.diagonal(Ar, br, c, cutoffs)
+    assert relerr(diagonal_hip(A, b, c, cutoffs, reordered=True).cpu().numpy(), ref) < TOL
+    cot = np.ones(cutoffs, complex)
+    dA_c, dB_c, dc = OD.diagonal_vjp(Ar, br, c, cutoffs, cot)
+    _, gA, gB, gc = diagonal_vjp_hip(Ar, br, c, cutoffs, cot, reordered=False)
+    assert np.max(np.abs(gA.cpu().numpy() - dA_c)) <= TOL * max(1.0, np.max(np.abs(dA_c)))
+    assert np.max(np.abs(gB.cpu().numpy() - dB_c)) <= TOL * max(1.0, np.max(np.abs(dB_c)))
+    assert abs(complex(gc) - dc) <= TOL * abs(dc)

@@ -157,3 +157,18 @@ def test_leftover_hip_errors():
     A9, b9, c9 = product_state(0.1, (0.1,) * 8)
     with pytest.raises(ValueError):
         leftover_hip(A9, b9, c9, 1, (1,) * 8)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("M,o,pnr", [(2, 3, (0,)), (3, 2, (0, 0)), (3, 0, (0, 3)), (2, 0, (0,))], ids=lambda x: str(x))
+def test_leftover_hip_trivial_boxes(M, o, pnr):
+    """Zero detected photons allowed (PNR cutoffs 0) and a one-entry leftover block."""
+    from oracle import leftover as OL
+
+    from mrmustard_amd.lattice.compact_fock import leftover_hip
+
+    A, b, c = random_dm(M, 95 + M)
+    ref = OL.leftover(A, b, c, o, pnr)
+    got = leftover_hip(A, b, c, o, pnr).cpu().numpy()
+    assert got.shape == ref.shape and relerr(got, ref) < TOL
