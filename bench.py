@@ -231,6 +231,9 @@ def main():
         G1 = torch.empty(shape1, dtype=torch.complex128, device=dev)
         t_dm = timed(lambda: st.vanilla_hip(shape1, A1t, b1t, c1t, out=G1), 20, 2)
         extra["dm_40x4_fwd_amps_per_s"] = G1.numel() / t_dm
+        # the all-pivot average (stable_numba, core.py:127-213) on the same lattice
+        t_st = timed(lambda: st.stable_hip(shape1, A1t, b1t, c1t, out=G1), 5, 1)
+        extra["dm_40x4_stable_fwd_amps_per_s"] = G1.numel() / t_st
         del G1
         # ---- 4-mode lossy density matrix, shape (10,)^8 = 1e8 amplitudes:
         #      forward, VJP of the stored G, and the VJP that recomputes G in a
