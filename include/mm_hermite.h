@@ -192,6 +192,19 @@ size_t mm_hermite_1leftover_workspace(int64_t output_cutoff, const int64_t *pnr_
 int mm_hermite_1leftover(const double *A, const double *b, const double *c, int64_t output_cutoff, const int64_t *pnr_cutoffs,
                          int Mt, int reordered, double *out, void *workspace, size_t ws_bytes, void *stream);
 
+/* Vector-Jacobian product of mm_hermite_1leftover: runs the forward (out is
+ * written as by mm_hermite_1leftover), then dA = sum dLdout * d out / dA, db,
+ * dc likewise, holomorphic (no conjugation), A's entries independent, dA and
+ * db in the caller's order.  Replaces grad_hermite_multidimensional_1leftoverMode
+ * (compactFock/inputValidation.py:121-141 -> singleLeftoverMode_grad.py:19-46,
+ * :50-225, :254-556, :560-692) with its contraction in
+ * hermite_renormalized_1leftoverMode_jax_bwd (jax_vjps/hermite.py:397-440),
+ * computed in reverse mode instead of the reference's Jacobian arrays. */
+size_t mm_hermite_1leftover_vjp_workspace(int64_t output_cutoff, const int64_t *pnr_cutoffs, int Mt);
+int mm_hermite_1leftover_vjp(const double *A, const double *b, const double *c, int64_t output_cutoff, const int64_t *pnr_cutoffs,
+                             int Mt, int reordered, const double *dLdout, double *out, double *dA, double *db, double *dc,
+                             void *workspace, size_t ws_bytes, void *stream);
+
 /* ------------------------------------------------------ contraction (MFMA)
  *
  * C = A * B for complex128 row-major matrices (A: M x K with row stride lda,

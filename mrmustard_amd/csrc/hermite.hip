@@ -27,6 +27,7 @@
 #include "diagonal.hpp"
 #include "diagonal_vjp.hpp"
 #include "leftover.hpp"
+#include "leftover_vjp.hpp"
 
 using namespace mm;
 
@@ -911,6 +912,101 @@ int launch_leftover(LeftArgs &a, const std::vector<int64_t> &counts, int num_cu,
     return MM_OK;
 }
 
+// Binds the forward's arrays to the workspace (layout of lo_geom).
+void lo_bind(const DiagGeom &g, int c0, const double *A, const double *b, const double *c, double *out, char *ws, DevState *ds,
+             LeftArgs &a) {
+    const int Mt = g.M;
+    a.t = g.a;
+    DiagArgs &t = a.t;
+    t.sq = ds->sq;
+    t.rsq = ds->rsq;
+    const int64_t C = int64_t(c0) * c0;
+    t.T = reinterpret_cast<int64_t *>(ws);
+    t.Q = t.T + int64_t(Mt + 1) * (t.S + 1);
+    t.list = reinterpret_cast<int32_t *>(ws + align256(size_t(2 * (Mt + 1)) * size_t(t.S + 1) * 8));
+    t.fwd = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(t.list) + align256(size_t(t.P) * 4));
+    t.inD = reinterpret_cast<double2 *>(ws + g.t_bytes);
+    t.inO = t.inD + 2 * Mt * t.P * C;
+    t.arr2 = t.inO + 2 * Mt * t.P * C;
+    t.a1010 = t.arr2 + g.n2 * C;
+    t.a1001 = t.a1010 + g.n11 * C;
+    t.arr0 = nullptr;
+    a.piv = reinterpret_cast<double2 *>(ws + g.bytes);
+    a.A = reinterpret_cast<const double2 *>(A);
+    a.B = reinterpret_cast<const double2 *>(b);
+    a.c = reinterpret_cast<const double2 *>(c);
+    a.c0 = c0;
+    a.C = int(C);
+    a.out = reinterpret_cast<double2 *>(out);
+}
+
+int lo_forward(LeftArgs &a, int Mt, int reordered, DevState *ds, hipStream_t s, std::vector<int64_t> *counts_out) {
+    if (diag_table_lds(a.t, Mt) == 0) {
+        hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(64), 0, s, a.t, Mt);
+        LAUNCH_CHECK("diag_counts_kernel");
+    }
+    const std::vector<int64_t> counts = diag_weight_counts(a.t, Mt);
+    int rc = MM_OK;
+    switch (Mt * 2 + (reordered ? 1 : 0)) {
+#define CASE(MM)                                                                   \
+    case 2 * MM: rc = launch_leftover<MM, false>(a, counts, ds->num_cu, s); break; \
+    case 2 * MM + 1: rc = launch_leftover<MM, true>(a, counts, ds->num_cu, s); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad number of detected modes");
+    }
+    if (rc) return rc;
+    LAUNCH_CHECK("lo kernels");
+    if (counts_out) *counts_out = counts;
+    return MM_OK;
+}
+
+// The VJP's blocks: one per PB groups (params, block entry) of the largest weight, at most 1024.
+int lo_vjp_blocks(const DiagGeom &g, int Mt, int64_t C) {
+    const std::vector<int64_t> counts = diag_weight_counts(g.a, Mt);
+    const int64_t mx = *std::max_element(counts.begin(), counts.end());
+    const int PB = DG_THREADS / (2 * Mt);
+    return int(std::max<int64_t>(1, std::min<int64_t>(1024, (mx * C + PB - 1) / PB)));
+}
+
+// forward workspace, adjoint mirrors of the inboxes and sub-lattices, adP, partial rows
+size_t lo_vjp_bytes(const DiagGeom &g, int Mt, int64_t C, size_t fwd_bytes, int nblk) {
+    const int GL = 2 * (Mt + 1);
+    const size_t mirror = size_t(16) * size_t(C) * size_t(4 * Mt * g.a.P + g.n2 + 2 * g.n11);
+    const size_t adp = size_t(16) * size_t(Mt) * size_t(g.a.P) * size_t(C);
+    return align256(fwd_bytes) + align256(mirror) + align256(adp) + size_t(16) * size_t(nblk + 1) * GL * (GL + 1);
+}
+
+template <int MT, bool REORDER>
+void launch_lo_vjp(LeftVjpArgs &v, const std::vector<int64_t> &counts, int nblk, int64_t nmirror, double2 *dA, double2 *dB,
+                   hipStream_t s) {
+    constexpr int PB = DG_THREADS / (2 * MT);
+    constexpr int GL = 2 * (MT + 1);
+    const int64_t C = v.a.C, P = v.a.t.P;
+    auto zero = [&](double2 *p, int64_t n) {
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256));
+        hipLaunchKernelGGL(zero_c128_span_kernel, dim3(unsigned(grid)), dim3(256), 0, s, p, n);
+    };
+    zero(v.adD, nmirror);
+    zero(v.adP, MT * P * C);
+    zero(v.part, int64_t(nblk + 1) * GL * (GL + 1));
+    std::vector<int64_t> starts(counts.size(), 0);
+    for (size_t w = 1; w < counts.size(); ++w) starts[w] = starts[w - 1] + counts[w - 1];
+    for (int w = v.a.t.S; w >= 0; --w) {
+        const int64_t groups = counts[w] * C;
+        if (groups == 0) continue;
+        // blocks 0 .. nw-1 of the fixed nblk slots: block b always adds to partial row b
+        const int nw = int(std::min<int64_t>(nblk, (groups + PB - 1) / PB));
+        const int64_t per = int64_t(nw) * PB;
+        const int iters = int((groups + per - 1) / per);
+        hipLaunchKernelGGL((lo_off_vjp_kernel<MT, REORDER>), dim3(nw), dim3(DG_THREADS), 0, s, v, counts[w], starts[w], iters);
+        hipLaunchKernelGGL((lo_diag_vjp_kernel<MT, REORDER>), dim3(nw), dim3(DG_THREADS), 0, s, v, counts[w], starts[w], iters);
+    }
+    hipLaunchKernelGGL((lo_init_vjp_kernel<MT, REORDER>), dim3(1), dim3(1024), 0, s, v, nblk);
+    hipLaunchKernelGGL((diag_vjp_finalize<MT + 1, REORDER>), dim3(GL * (GL + 1)), dim3(256), 0, s, v.part, nblk + 1, dA, dB);
+}
+
 struct GlobalJob {
     Shape S;
     int64_t a_off, b_off, c_off, g_off;
@@ -1455,46 +1551,65 @@ int mm_hermite_1leftover(const double *A, const double *b, const double *c, int6
     rc = device_state(&ds);
     if (rc) return rc;
     LeftArgs a;
-    a.t = g.a;
-    DiagArgs &t = a.t;
-    t.sq = ds->sq;
-    t.rsq = ds->rsq;
-    char *ws = reinterpret_cast<char *>(workspace);
+    lo_bind(g, c0, A, b, c, out, reinterpret_cast<char *>(workspace), ds, a);
+    std::vector<int64_t> counts;
+    return lo_forward(a, Mt, reordered, ds, reinterpret_cast<hipStream_t>(stream), &counts);
+}
+
+size_t mm_hermite_1leftover_vjp_workspace(int64_t output_cutoff, const int64_t *pnr_cutoffs, int Mt) {
+    DiagGeom g;
+    int c0 = 0;
+    size_t bytes = 0;
+    if (lo_geom(output_cutoff, pnr_cutoffs, Mt, &g, &c0, &bytes)) return 0;
+    return lo_vjp_bytes(g, Mt, int64_t(c0) * c0, bytes, lo_vjp_blocks(g, Mt, int64_t(c0) * c0));
+}
+
+int mm_hermite_1leftover_vjp(const double *A, const double *b, const double *c, int64_t output_cutoff, const int64_t *pnr_cutoffs,
+                             int Mt, int reordered, const double *dLdout, double *out, double *dA, double *db, double *dc,
+                             void *workspace, size_t ws_bytes, void *stream) {
+    DiagGeom g;
+    int c0 = 0;
+    size_t fwd_bytes = 0;
+    int rc = lo_geom(output_cutoff, pnr_cutoffs, Mt, &g, &c0, &fwd_bytes);
+    if (rc) return rc;
+    if (!A || !b || !c || !dLdout || !out || !dA || !db || !dc) return fail(MM_ERR_INVALID, "NULL array pointer");
     const int64_t C = int64_t(c0) * c0;
-    t.T = reinterpret_cast<int64_t *>(ws);
-    t.Q = t.T + int64_t(Mt + 1) * (t.S + 1);
-    t.list = reinterpret_cast<int32_t *>(ws + align256(size_t(2 * (Mt + 1)) * size_t(t.S + 1) * 8));
-    t.fwd = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(t.list) + align256(size_t(t.P) * 4));
-    t.inD = reinterpret_cast<double2 *>(ws + g.t_bytes);
-    t.inO = t.inD + 2 * Mt * t.P * C;
-    t.arr2 = t.inO + 2 * Mt * t.P * C;
-    t.a1010 = t.arr2 + g.n2 * C;
-    t.a1001 = t.a1010 + g.n11 * C;
-    t.arr0 = nullptr;
-    a.piv = reinterpret_cast<double2 *>(ws + g.bytes);
-    a.A = reinterpret_cast<const double2 *>(A);
-    a.B = reinterpret_cast<const double2 *>(b);
-    a.c = reinterpret_cast<const double2 *>(c);
-    a.c0 = c0;
-    a.C = int(C);
-    a.out = reinterpret_cast<double2 *>(out);
+    const int nblk = lo_vjp_blocks(g, Mt, C);
+    const size_t need = lo_vjp_bytes(g, Mt, C, fwd_bytes, nblk);
+    if (!workspace || ws_bytes < need)
+        return fail(MM_ERR_WORKSPACE, "1leftover VJP workspace of %zu bytes needed, %zu given", need, ws_bytes);
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    char *ws = reinterpret_cast<char *>(workspace);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (diag_table_lds(t, Mt) == 0) {
-        hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(64), 0, s, t, Mt);
-        LAUNCH_CHECK("diag_counts_kernel");
-    }
-    const std::vector<int64_t> counts = diag_weight_counts(t, Mt);
+    LeftVjpArgs v;
+    lo_bind(g, c0, A, b, c, out, ws, ds, v.a);
+    std::vector<int64_t> counts;
+    rc = lo_forward(v.a, Mt, reordered, ds, s, &counts);
+    if (rc) return rc;
+    const int64_t P = g.a.P;
+    const int64_t nmirror = C * (4 * Mt * P + g.n2 + 2 * g.n11);
+    v.cot = reinterpret_cast<const double2 *>(dLdout);
+    v.adD = reinterpret_cast<double2 *>(ws + align256(fwd_bytes));
+    v.adO = v.adD + 2 * Mt * P * C;
+    v.ad2 = v.adO + 2 * Mt * P * C;
+    v.ad1010 = v.ad2 + g.n2 * C;
+    v.ad1001 = v.ad1010 + g.n11 * C;
+    v.adP = reinterpret_cast<double2 *>(ws + align256(fwd_bytes) + align256(size_t(16) * size_t(nmirror)));
+    v.part = reinterpret_cast<double2 *>(reinterpret_cast<char *>(v.adP) + align256(size_t(16) * size_t(Mt * P * C)));
+    v.dc = reinterpret_cast<double2 *>(dc);
+    double2 *dA2 = reinterpret_cast<double2 *>(dA), *dB2 = reinterpret_cast<double2 *>(db);
     switch (Mt * 2 + (reordered ? 1 : 0)) {
-#define CASE(MM)                                                             \
-    case 2 * MM: rc = launch_leftover<MM, false>(a, counts, ds->num_cu, s); break; \
-    case 2 * MM + 1: rc = launch_leftover<MM, true>(a, counts, ds->num_cu, s); break;
+#define CASE(MM)                                                                        \
+    case 2 * MM: launch_lo_vjp<MM, false>(v, counts, nblk, nmirror, dA2, dB2, s); break; \
+    case 2 * MM + 1: launch_lo_vjp<MM, true>(v, counts, nblk, nmirror, dA2, dB2, s); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
         default:
             return fail(MM_ERR_INVALID, "bad number of detected modes");
     }
-    if (rc) return rc;
-    LAUNCH_CHECK("lo kernels");
+    LAUNCH_CHECK("lo vjp kernels");
     return MM_OK;
 }
 

@@ -89,14 +89,14 @@ struct LoLane {
 };
 
 template <int MT, int PB>
-__device__ __forceinline__ LoLane<MT> lo_lane(const LeftArgs &a, int64_t count, int64_t start) {
+__device__ __forceinline__ LoLane<MT> lo_lane(const LeftArgs &a, int64_t count, int64_t start, int64_t blk) {
     LoLane<MT> l;
     const int t = threadIdx.x;
     l.slot = t % PB;
     l.i = t / PB;
     l.m = l.i >> 1;
     l.lane_ok = l.i < 2 * MT;
-    const int64_t grp = int64_t(blockIdx.x) * PB + l.slot;
+    const int64_t grp = blk * PB + l.slot;
     l.act = l.lane_ok && grp < count * a.C;
     l.f = l.gs = l.sm = 0;
     l.fw = -1;
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(DG_THREADS) void lo_diag_kernel(LeftArgs a, int w, 
     __shared__ double2 sA[2 * L][2 * L];
     for (int e = threadIdx.x; e < 4 * L * L; e += DG_THREADS)
         sA[e / (2 * L)][e % (2 * L)] = a.A[src_index<L, REORDER>(e / (2 * L)) * 2 * L + src_index<L, REORDER>(e % (2 * L))];
-    const LoLane<MT> l = lo_lane<MT, PB>(a, count, start);
+    const LoLane<MT> l = lo_lane<MT, PB>(a, count, start, blockIdx.x);
     const int64_t P = a.t.P, C = a.C;
     const int runmask = lo_runmask<MT>(a, l);
     const bool diag = l.act && (a.t.cut[0] == 1 || l.p[0] < a.t.cut[0] - 1);
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(DG_THREADS) void lo_off_kernel(LeftArgs a, int w, i
     __shared__ double2 sA[2 * L][2 * L];
     for (int e = threadIdx.x; e < 4 * L * L; e += DG_THREADS)
         sA[e / (2 * L)][e % (2 * L)] = a.A[src_index<L, REORDER>(e / (2 * L)) * 2 * L + src_index<L, REORDER>(e % (2 * L))];
-    const LoLane<MT> l = lo_lane<MT, PB>(a, count, start);
+    const LoLane<MT> l = lo_lane<MT, PB>(a, count, start, blockIdx.x);
     const int64_t P = a.t.P, C = a.C;
     const int runmask = lo_runmask<MT>(a, l);
     double2 g0 = make_double2(0.0, 0.0);

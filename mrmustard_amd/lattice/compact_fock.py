@@ -22,7 +22,7 @@ import torch
 from .. import _lib
 from .strategies import C128, _ptr, _shape_arr, _stream, _workspace, as_device_c128
 
-__all__ = ["diagonal_hip", "diagonal_vjp_hip", "input_validation", "leftover_hip"]
+__all__ = ["diagonal_hip", "diagonal_vjp_hip", "input_validation", "leftover_hip", "leftover_vjp_hip"]
 
 
 def input_validation(A, rtol: float = 1e-05, atol: float = 1e-08) -> bool:
@@ -138,3 +138,38 @@ def leftover_hip(A, b, c, output_cutoff: int, pnr_cutoffs, reordered: bool = Tru
     )
     _lib.check(rc, "mm_hermite_1leftover")
     return out
+
+
+def leftover_vjp_hip(A, b, c, output_cutoff: int, pnr_cutoffs, dLdout, reordered: bool = True):
+    """(out, dA, db, dc): the one-leftover-mode forward and its holomorphic VJP
+    for the cotangent ``dLdout`` (shape of out, not conjugated), every entry of
+    A independent, in the order A and b are given in.  The reference forms the
+    Jacobians (compactFock/singleLeftoverMode_grad.py:695-724, via
+    inputValidation.py:121-141) and contracts them with the cotangent
+    (jax_vjps/hermite.py:397-440); this computes the contraction directly with
+    a reverse sweep."""
+    A, b, c = as_device_c128(A), as_device_c128(b), as_device_c128(c)
+    pnr = tuple(int(k) for k in pnr_cutoffs)
+    M = len(pnr) + 1
+    if M <= 1:
+        raise ValueError("The number of modes should be greater than 1.")
+    if tuple(A.shape) != (2 * M, 2 * M) or tuple(b.shape) != (2 * M,) or c.numel() != 1:
+        raise ValueError(f"A must be ({2 * M}, {2 * M}), b ({2 * M},) and c scalar for {M} modes")
+    o = int(output_cutoff)
+    shape = (o + 1, o + 1, *(k + 1 for k in pnr))
+    g = as_device_c128(dLdout)
+    if tuple(g.shape) != shape:
+        raise ValueError(f"dLdout has shape {tuple(g.shape)}, expected {shape}")
+    out = torch.empty(shape, dtype=C128, device=A.device)
+    dA = torch.empty((2 * M, 2 * M), dtype=C128, device=A.device)
+    db = torch.empty((2 * M,), dtype=C128, device=A.device)
+    dc = torch.empty((), dtype=C128, device=A.device)
+    s = _shape_arr(pnr)
+    L = _lib.load()
+    ws = _workspace(L.mm_hermite_1leftover_vjp_workspace(o, s.ctypes.data, M - 1))
+    rc = L.mm_hermite_1leftover_vjp(
+        _ptr(A), _ptr(b), _ptr(c), o, s.ctypes.data, M - 1, int(bool(reordered)), _ptr(g), _ptr(out), _ptr(dA), _ptr(db),
+        _ptr(dc), _ptr(ws), 0 if ws is None else ws.numel(), _stream(),
+    )
+    _lib.check(rc, "mm_hermite_1leftover_vjp")
+    return out, dA, db, dc

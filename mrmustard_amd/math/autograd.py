@@ -119,3 +119,35 @@ class HermiteDiagonal(torch.autograd.Function):
             None,
             None,
         )
+
+
+class HermiteLeftover(torch.autograd.Function):
+    """out = hermite_renormalized_1leftoverMode(A, b, c, output_cutoff,
+    pnr_cutoffs) with the reverse-sweep VJP (the reference's jax custom VJP
+    pair, jax_vjps/hermite.py:343-445).  The backward recomputes the forward
+    together with the VJP (``leftover_vjp_hip``), so only (A, b, c) are kept."""
+
+    @staticmethod
+    def forward(ctx, A, b, c, output_cutoff, pnr_cutoffs, reordered):
+        from ..lattice import compact_fock as cf
+
+        out = cf.leftover_hip(A, b, c, output_cutoff, pnr_cutoffs, reordered=reordered)
+        ctx.save_for_backward(A, b, c)
+        ctx.args = (int(output_cutoff), tuple(int(k) for k in pnr_cutoffs), bool(reordered))
+        return out
+
+    @staticmethod
+    def backward(ctx, grad):
+        from ..lattice import compact_fock as cf
+
+        A, b, c = ctx.saved_tensors
+        o, pnr, reordered = ctx.args
+        _, dA, db, dc = cf.leftover_vjp_hip(A, b, c, o, pnr, grad.conj().resolve_conj(), reordered=reordered)
+        return (
+            dA.conj().resolve_conj().reshape(A.shape),
+            db.conj().resolve_conj().reshape(b.shape),
+            dc.conj().resolve_conj().reshape(c.shape),
+            None,
+            None,
+            None,
+        )

@@ -24,7 +24,13 @@ import torch
 from ..lattice import compact_fock as cf
 from ..lattice import strategies as st
 from ..settings import settings
-from .autograd import HermiteDiagonal, HermiteRenormalized, HermiteRenormalizedBatched, HermiteRenormalizedRecompute
+from .autograd import (
+    HermiteDiagonal,
+    HermiteLeftover,
+    HermiteRenormalized,
+    HermiteRenormalizedBatched,
+    HermiteRenormalizedRecompute,
+)
 
 
 def _needs_grad(*xs) -> bool:
@@ -132,9 +138,12 @@ class BackendHip:
                                            reorderedAB: bool = True):
         """Conditional density matrices of mode 0 (backend_jax.py:524-534 ->
         jax_vjps/hermite.py:343-387; the numpy backend's fast_diagonal,
-        backend_numpy.py:434-446, returns the same amplitudes)."""
+        backend_numpy.py:434-446, returns the same amplitudes).  With gradients
+        tracked it goes through the autograd Function, as the JAX backend's
+        custom VJP does (jax_vjps/hermite.py:390-445)."""
         if _needs_grad(A, b, c):
-            raise NotImplementedError("gradients of hermite_renormalized_1leftoverMode are not implemented on the GPU")
+            A, b, c = st.as_device_c128(A), st.as_device_c128(b), st.as_device_c128(c)
+            return HermiteLeftover.apply(A, b, c, int(output_cutoff), tuple(int(k) for k in pnr_cutoffs), bool(reorderedAB))
         return cf.leftover_hip(A, b, c, output_cutoff, pnr_cutoffs, reordered=reorderedAB)
 
     def hermite_renormalized_batched(self, A, b, c, shape: tuple[int], stable: bool = False, out=None):

@@ -172,3 +172,141 @@ def test_leftover_hip_trivial_boxes(M, o, pnr):
     ref = OL.leftover(A, b, c, o, pnr)
     got = leftover_hip(A, b, c, o, pnr).cpu().numpy()
     assert got.shape == ref.shape and relerr(got, ref) < TOL
+
+
+# ----------------------------------------------------------------- VJP
+
+VJP_CASES = [(2, 3, (4,), 11), (3, 3, (1, 2), 12), (3, 2, (3, 2), 13), (4, 2, (2, 1, 2), 14), (2, 0, (3,), 15), (3, 3, (0, 2), 16)]
+
+
+def caller_order(dA_c, dB_c, M):
+    """Compact-order gradients mapped to the [modes..., modes...] order (inverse of reorder_ab)."""
+    order = np.arange(2 * M).reshape(2, -1).T.flatten()
+    dA = np.zeros_like(dA_c)
+    dA[np.ix_(order, order)] = dA_c
+    dB = np.zeros_like(dB_c)
+    dB[order] = dB_c
+    return dA, dB
+
+
+@pytest.mark.parametrize("M,o,pnr,seed", VJP_CASES[:4])
+def test_oracle_vjp_vs_finite_differences(M, o, pnr, seed):
+    """The forward-mode Jacobian (singleLeftoverMode_grad.py) against central
+    differences of the pinned forward, every entry of A and b perturbed on its
+    own, and c; a dropped or misplaced derivative term fails here."""
+    from oracle import diagonal as OD
+    from oracle import leftover as OL
+
+    A, b, c = random_dm(M, seed)
+    Ar, br = OD.reorder_ab(A, b)
+    cut = (o + 1,) + tuple(k + 1 for k in pnr)
+    val, _, _ = OL.leftover_jacobian(Ar, br, c, cut)
+    assert relerr(val, OL.leftover_compact(Ar, br, c, cut)) < 1e-14
+    rng = np.random.default_rng(seed)
+    cot = rng.standard_normal(val.shape) + 1j * rng.standard_normal(val.shape)
+    dA, dB, dc = OL.leftover_vjp(Ar, br, c, cut, cot)
+
+    def loss(A_, B_, c_):
+        return np.sum(cot * OL.leftover_compact(A_, B_, c_, cut))
+
+    h, G = 1e-6, 2 * M
+    fdA = np.zeros((G, G), complex)
+    for i in range(G):
+        for j in range(G):
+            E = np.zeros((G, G), complex)
+            E[i, j] = h
+            fdA[i, j] = (loss(Ar + E, br, c) - loss(Ar - E, br, c)) / (2 * h)
+    fdB = np.array([(loss(Ar, br + h * np.eye(G)[i], c) - loss(Ar, br - h * np.eye(G)[i], c)) / (2 * h) for i in range(G)])
+    fdc = (loss(Ar, br, c + h) - loss(Ar, br, c - h)) / (2 * h)
+    assert relerr(dA, fdA) < 1e-8 and relerr(dB, fdB) < 1e-8 and abs(dc - fdc) < 1e-8 * abs(dc)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("M,o,pnr,seed", VJP_CASES + [(2, 6, (9,), 17), (3, 4, (5, 4), 18), (5, 1, (2, 1, 2, 1), 19)],
+                         ids=lambda x: str(x))
+def test_leftover_vjp_hip_vs_oracle(M, o, pnr, seed):
+    from oracle import diagonal as OD
+    from oracle import leftover as OL
+
+    from mrmustard_amd.lattice.compact_fock import leftover_vjp_hip
+
+    A, b, c = random_dm(M, seed)
+    Ar, br = OD.reorder_ab(A, b)
+    cut = (o + 1,) + tuple(k + 1 for k in pnr)
+    rng = np.random.default_rng(seed)
+    shape = (o + 1,) + cut
+    cot = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    print(f"1leftover VJP M={M} output_cutoff={o} pnr={pnr}", flush=True)
+    dA_c, dB_c, dc = OL.leftover_vjp(Ar, br, c, cut, cot)
+    out, gA, gB, gc = leftover_vjp_hip(Ar, br, c, o, pnr, cot, reordered=False)
+    assert relerr(out.cpu().numpy(), OL.leftover_compact(Ar, br, c, cut)) < TOL
+    assert relerr(gA.cpu().numpy(), dA_c) < TOL and relerr(gB.cpu().numpy(), dB_c) < TOL
+    assert abs(complex(gc) - dc) <= TOL * abs(dc)
+    dA, dB = caller_order(dA_c, dB_c, M)
+    _, gA, gB, gc = leftover_vjp_hip(A, b, c, o, pnr, cot, reordered=True)
+    assert relerr(gA.cpu().numpy(), dA) < TOL and relerr(gB.cpu().numpy(), dB) < TOL
+    # deterministic: a second call gives the same bits
+    _, gA2, gB2, _ = leftover_vjp_hip(A, b, c, o, pnr, cot, reordered=True)
+    assert np.array_equal(gA.cpu().numpy(), gA2.cpu().numpy()) and np.array_equal(gB.cpu().numpy(), gB2.cpu().numpy())
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+def test_leftover_autograd_convention():
+    """torch grads = conj(VJP_ref(conj(grad_out))) through
+    math.hermite_renormalized_1leftoverMode (the reference's JAX custom VJP is
+    holomorphic)."""
+    import torch
+
+    from oracle import diagonal as OD
+    from oracle import leftover as OL
+
+    from mrmustard_amd import math as mm
+
+    M, o, pnr = 3, 3, (2, 3)
+    A, b, c = random_dm(M, 21)
+    At = torch.tensor(np.asarray(A), device="cuda", requires_grad=True)
+    bt = torch.tensor(np.asarray(b), device="cuda", requires_grad=True)
+    ct = torch.tensor(complex(c), dtype=torch.complex128, device="cuda", requires_grad=True)
+    out = mm.hermite_renormalized_1leftoverMode(At, bt, ct, o, pnr)
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(tuple(out.shape)) + 1j * rng.standard_normal(tuple(out.shape))
+    torch.real(torch.sum(torch.tensor(w, device="cuda") * out)).backward()
+    Ar, br = OD.reorder_ab(A, b)
+    dA_c, dB_c, dc = OL.leftover_vjp(Ar, br, c, (o + 1,) + tuple(k + 1 for k in pnr), w)
+    dA, dB = caller_order(dA_c, dB_c, M)
+    assert relerr(At.grad.cpu().numpy(), np.conj(dA)) < TOL
+    assert relerr(bt.grad.cpu().numpy(), np.conj(dB)) < TOL
+    assert abs(complex(ct.grad) - np.conj(dc)) <= TOL * abs(dc)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("M,o,pnr", [(3, 20, (14, 14)), (2, 100, (4,))], ids=lambda x: str(x))
+def test_leftover_vjp_large_vs_finite_differences(M, o, pnr):
+    """Sizes beyond the oracle (the bench's 21 x 21 x 15 x 15, a 101 x 101
+    block): the VJP of the real loss sum(w * out) in random directions of
+    (A, b, c) against central differences of the GPU forward, A perturbed
+    symmetrically (a physical direction)."""
+    from mrmustard_amd.lattice.compact_fock import leftover_hip, leftover_vjp_hip
+
+    A, b, c = random_dm(M, 40 + M)
+    shape = (o + 1, o + 1, *(k + 1 for k in pnr))
+    rng = np.random.default_rng(6)
+    w = rng.standard_normal(shape)
+    _, gA, gB, gc = leftover_vjp_hip(A, b, c, o, pnr, w.astype(complex))
+    gA, gB, gc = gA.cpu().numpy(), gB.cpu().numpy(), complex(gc)
+
+    def loss(A_, b_, c_):
+        return np.sum(w * leftover_hip(A_, b_, c_, o, pnr).cpu().numpy())
+
+    h, G = 1e-6, 2 * M
+    for k in range(3):
+        dA = rng.standard_normal((G, G)) + 1j * rng.standard_normal((G, G))
+        dA = (dA + dA.T) / 2
+        db = rng.standard_normal(G) + 1j * rng.standard_normal(G)
+        dc = complex(rng.standard_normal(), rng.standard_normal())
+        fd = (loss(A + h * dA, b + h * db, c + h * dc) - loss(A - h * dA, b - h * db, c - h * dc)) / (2 * h)
+        an = np.sum(gA * dA) + np.sum(gB * db) + gc * dc
+        assert abs(an - fd) < 1e-6 * max(1.0, abs(fd)), (k, an, fd)
