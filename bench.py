@@ -101,6 +101,12 @@ def main():
         action="store_true",
         help="only run the diagonal strategy on the 6-mode density matrix, cutoffs 10, three times (for counters)",
     )
+    ap.add_argument(
+        "--leftover-probe",
+        action="store_true",
+        help="only run the one-leftover-mode forward and its VJP on the 3-mode density matrix "
+             "(20 x 20 x 15 x 15), three times each (for kernel traces)",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
@@ -131,6 +137,19 @@ def main():
             diagonal_hip(Adt, bdt, cdt, (10,) * 6, reordered=True)
         torch.cuda.synchronize()
         print(json.dumps({"diagonal_probe_values_written": diag_values_written((10,) * 6), "calls": 3}))
+        return
+    if args.leftover_probe:
+        from mrmustard_amd.lattice.compact_fock import leftover_hip, leftover_vjp_hip
+
+        Al, bl, cl = W.pnr_dm(3, seed=7)
+        Alt, blt, clt = (torch.as_tensor(np.asarray(x), device=dev) for x in (Al, bl, cl))
+        cot = torch.ones((20, 20, 15, 15), dtype=torch.complex128, device=dev)
+        for _ in range(3):
+            leftover_hip(Alt, blt, clt, 19, (14, 14))
+        for _ in range(3):
+            leftover_vjp_hip(Alt, blt, clt, 19, (14, 14), cot)
+        torch.cuda.synchronize()
+        print(json.dumps({"leftover_probe_calls": 3, "vjp_calls": 3}))
         return
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
 
@@ -365,12 +384,18 @@ def main():
         o_l, pnr_l = 19, (14, 14)
         t_l = timed(lambda: leftover_hip(Alt, blt, clt, o_l, pnr_l), 10, 2)
         n_l = (o_l + 1) ** 2 * int(np.prod([k + 1 for k in pnr_l]))
+        from mrmustard_amd.lattice.compact_fock import leftover_vjp_hip
+
+        cot_l = torch.ones((o_l + 1, o_l + 1, *(k + 1 for k in pnr_l)), dtype=torch.complex128, device=dev)
+        t_lv = timed(lambda: leftover_vjp_hip(Alt, blt, clt, o_l, pnr_l, cot_l), 5, 1)
         extra["leftover_3modes_20x20x15x15"] = {
             "entries_per_s": n_l / t_l,
             "ms_per_call": t_l * 1e3,
             "entries": n_l,
             "launches": 2 * (sum(pnr_l) + 1) + 2,
-            "note": "API call: input validation, workspace, two launches per total detected photon number",
+            "fwd_plus_vjp_ms_per_call": t_lv * 1e3,
+            "note": "API call: input validation, workspace, two launches per total detected photon number; the VJP "
+                    "call runs the forward again, then two reverse launches per weight",
         }
 
     cpu = None

@@ -11,7 +11,10 @@ pnr_cutoffs + 1) shape convention of hermite_renormalized_1leftoverMode
 (jax_vjps/hermite.py:343-387, backend_numpy.py:434-446).  Only ``tests/`` may
 import it.  Pinned in ``tests/test_leftover.py`` against the full lattice of the
 pinned vanilla oracle (test_compactFock.py:46-69, test_fast_diagonal.py:24-45)
-and against product-state closed forms.
+and against product-state closed forms.  ``leftover_jacobian`` / ``leftover_vjp``
+restate the forward-mode gradient (singleLeftoverMode_grad.py) on dual numbers
+and are pinned there by central differences of ``leftover_compact``, per entry
+of A, B and c.
 """
 
 from __future__ import annotations
