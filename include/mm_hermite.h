@@ -205,6 +205,22 @@ int mm_hermite_1leftover_vjp(const double *A, const double *b, const double *c, 
                              int Mt, int reordered, const double *dLdout, double *out, double *dA, double *db, double *dc,
                              void *workspace, size_t ws_bytes, void *stream);
 
+/* The Fock array of a triple with derived variables, the lattice never
+ * stored whole: out[h] = sum_k G[h, k] w[k], G the lattice of (A, b, c) on
+ * `shape` (D axes), h its first n_head axes, k the other D - n_head axes
+ * flattened (K = prod(shape[n_head:]) entries of w; out has prod(shape[:n_head])
+ * entries).  Replaces CircuitComponent.fock_array's num_derived_vars > 0
+ * branch (mrmustard/lab/circuit_components.py:516-528: the lattice of
+ * (A, b, 1) on shape + shape_derived_vars, then math.einsum with c over the
+ * derived indices) for one triple: the lattice is computed one level-0 slab
+ * at a time in a three-slab window (3 * prod(shape[1:]) * 16 bytes of
+ * workspace) and each slab is contracted as soon as it exists.  Lattices that
+ * fit in LDS are computed whole.  The values of G are those of
+ * mm_hermite_fwd; the sums over k are in a fixed order of their own. */
+size_t mm_hermite_fwd_contract_workspace(const int64_t *shape, int D);
+int mm_hermite_fwd_contract(const double *A, const double *b, const double *c, const int64_t *shape, int D, int n_head,
+                            const double *w, double *out, void *workspace, size_t ws_bytes, void *stream);
+
 /* ------------------------------------------------------ contraction (MFMA)
  *
  * C = A * B for complex128 row-major matrices (A: M x K with row stride lda,

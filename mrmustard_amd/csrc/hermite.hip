@@ -28,6 +28,7 @@
 #include "diagonal_vjp.hpp"
 #include "leftover.hpp"
 #include "leftover_vjp.hpp"
+#include "rows_dot.hpp"
 
 using namespace mm;
 
@@ -560,6 +561,74 @@ size_t recompute_ws_bytes(const Shape &S, const DevState *ds) {
     return 3 * align256(size_t(S.st[0]) * 16) + size_t(recompute_blocks(S, ds)) * nacc_of(S.D) * 16 + 256;
 }
 
+// The face {k_0 = 0} of the lattice into `face` (stride[0] entries): the LDS
+// sweep from a level `top` >= 1 (it writes only flat indices below
+// stride[top-1] <= stride[0]), then slab launches for levels top-1 .. 1.
+int window_face(const Shape &S, const double *A, const double *b, const double *c, double2 *face, const DevState *ds,
+                hipStream_t s) {
+    int T = 64;
+    int top = choose_top(S, false, ds->max_lds, lds_maxt(S.D, false), &T);
+    if (top == 0) {
+        top = 1;
+        T = team_threads(S.st[1], lds_maxt(S.D, false));
+        if (lds_bytes(S, 1, false, T / MM_WAVE, kernel_dmax(S.D)) > ds->max_lds || !packed_bits_ok(S, 1))
+            return fail(MM_ERR_UNSUPPORTED, "slab window: face does not fit in LDS");
+    }
+    LdsArgs a;
+    memset(&a, 0, sizeof(a));
+    a.P = bases(A, b, c, nullptr, reinterpret_cast<double *>(face), nullptr, nullptr, nullptr, nullptr);
+    a.T.sq = ds->sq;
+    a.T.rsq = ds->rsq;
+    a.mode = MODE_UNIFORM;
+    a.n = 1;
+    a.tmpl = make_job(S, top, KIND_BLOCK);
+    int rc = launch_lds(a, S.D, false, 1, T, lds_bytes(S, top, false, T / MM_WAVE, kernel_dmax(S.D)), s);
+    if (rc) return rc;
+    SlabArgs sa;
+    sa.g = geom_of(S);
+    sa.A = reinterpret_cast<const double2 *>(A);
+    sa.b = reinterpret_cast<const double2 *>(b);
+    sa.G = face;
+    sa.sq = ds->sq;
+    sa.rsq = ds->rsq;
+    for (int d = top - 1; d >= 1; --d) {
+        for (int m = 1; m < S.s[d]; ++m) {
+            sa.d = d;
+            sa.m = m;
+            rc = launch_slab(sa, S.D, S.st[d], ds->num_cu, s);
+            if (rc) return rc;
+        }
+    }
+    return MM_OK;
+}
+
+// Level-0 slab m (m >= 1) into cur from the two slabs below it.
+int window_slab0(const Shape &S, const double *A, const double *b, double2 *cur, double2 *p1, double2 *p2, int m,
+                 const DevState *ds, hipStream_t s) {
+    const int64_t S0 = S.st[0];
+    const int64_t fblocks = std::max<int64_t>(1, std::min<int64_t>((S0 + 255) / 256, int64_t(ds->num_cu) * 16));
+    Slab0Args f;
+    f.g = geom_of(S);
+    f.A = reinterpret_cast<const double2 *>(A);
+    f.b = reinterpret_cast<const double2 *>(b);
+    f.p1 = p1;
+    f.p2 = p2;
+    f.cur = cur;
+    f.sq = ds->sq;
+    f.rsq = ds->rsq;
+    f.m = m;
+    switch (S.D) {
+#define CASE(DD) \
+    case DD: hipLaunchKernelGGL((fwd_slab0_kernel<DD>), dim3(unsigned(fblocks)), dim3(256), 0, s, f); break;
+        CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad D");
+    }
+    LAUNCH_CHECK("fwd_slab0_kernel");
+    return MM_OK;
+}
+
 int vjp_recompute(const Shape &S, const double *A, const double *b, const double *c, const double *g, double *dA, double *db,
                   double *dc, void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s) {
     const size_t need = recompute_ws_bytes(S, ds);
@@ -579,65 +648,15 @@ int vjp_recompute(const Shape &S, const double *A, const double *b, const double
     hipLaunchKernelGGL(zero_c128_kernel, dim3(64), dim3(256), 0, s, partial, int64_t(nblk) * NA);
     LAUNCH_CHECK("zero_c128_kernel");
 
-    // 1. the face {k_0 = 0} into slot 0: LDS sweep from `top` >= 1 (it writes
-    //    only flat indices below stride[top-1] <= stride[0]), then slab
-    //    launches for levels top-1 .. 1
-    int top = choose_top(S, false, ds->max_lds, lds_maxt(S.D, false), &T);
-    if (top == 0) {
-        top = 1;
-        T = team_threads(S.st[1], lds_maxt(S.D, false));
-        if (lds_bytes(S, 1, false, T / MM_WAVE, kernel_dmax(S.D)) > ds->max_lds || !packed_bits_ok(S, 1))
-            return fail(MM_ERR_UNSUPPORTED, "recompute: face does not fit in LDS");
-    }
-    LdsArgs a;
-    memset(&a, 0, sizeof(a));
-    a.P = bases(A, b, c, nullptr, reinterpret_cast<double *>(slot[0]), nullptr, nullptr, nullptr, nullptr);
-    a.T.sq = ds->sq;
-    a.T.rsq = ds->rsq;
-    a.mode = MODE_UNIFORM;
-    a.n = 1;
-    a.tmpl = make_job(S, top, KIND_BLOCK);
-    int rc = launch_lds(a, S.D, false, 1, T, lds_bytes(S, top, false, T / MM_WAVE, kernel_dmax(S.D)), s);
+    // 1. the face {k_0 = 0} into slot 0
+    int rc = window_face(S, A, b, c, slot[0], ds, s);
     if (rc) return rc;
-    SlabArgs sa;
-    sa.g = geom_of(S);
-    sa.A = reinterpret_cast<const double2 *>(A);
-    sa.b = reinterpret_cast<const double2 *>(b);
-    sa.G = slot[0];
-    sa.sq = ds->sq;
-    sa.rsq = ds->rsq;
-    for (int d = top - 1; d >= 1; --d) {
-        for (int m = 1; m < S.s[d]; ++m) {
-            sa.d = d;
-            sa.m = m;
-            rc = launch_slab(sa, S.D, S.st[d], ds->num_cu, s);
-            if (rc) return rc;
-        }
-    }
     // 2. level 0: VJP of slab 0, then recompute slab m and add its VJP terms
     const int64_t S0 = S.st[0];
-    const int64_t fblocks = std::max<int64_t>(1, std::min<int64_t>((S0 + 255) / 256, int64_t(ds->num_cu) * 16));
     for (int m = 0; m < S.s[0]; ++m) {
         if (m >= 1) {
-            Slab0Args f;
-            f.g = geom_of(S);
-            f.A = reinterpret_cast<const double2 *>(A);
-            f.b = reinterpret_cast<const double2 *>(b);
-            f.p1 = slot[(m - 1) % 3];
-            f.p2 = m >= 2 ? slot[(m - 2) % 3] : slot[(m - 1) % 3];
-            f.cur = slot[m % 3];
-            f.sq = ds->sq;
-            f.rsq = ds->rsq;
-            f.m = m;
-            switch (S.D) {
-#define CASE(DD) \
-    case DD: hipLaunchKernelGGL((fwd_slab0_kernel<DD>), dim3(unsigned(fblocks)), dim3(256), 0, s, f); break;
-                CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-#undef CASE
-                default:
-                    return fail(MM_ERR_INVALID, "bad D");
-            }
-            LAUNCH_CHECK("fwd_slab0_kernel");
+            rc = window_slab0(S, A, b, slot[m % 3], slot[(m - 1) % 3], m >= 2 ? slot[(m - 2) % 3] : slot[(m - 1) % 3], m, ds, s);
+            if (rc) return rc;
         }
         SlabVjpArgs v;
         v.g = geom_of(S);
@@ -1005,6 +1024,35 @@ void launch_lo_vjp(LeftVjpArgs &v, const std::vector<int64_t> &counts, int nblk,
     }
     hipLaunchKernelGGL((lo_init_vjp_kernel<MT, REORDER>), dim3(1), dim3(1024), 0, s, v, nblk);
     hipLaunchKernelGGL((diag_vjp_finalize<MT + 1, REORDER>), dim3(GL * (GL + 1)), dim3(256), 0, s, v.part, nblk + 1, dA, dB);
+}
+
+// ------------------------------------------- conversion fused with the contraction
+
+int launch_rows_dot(const double2 *G, const double2 *w, double2 *out, int64_t rows, int64_t K, hipStream_t s) {
+    if (rows <= 0) return MM_OK;
+    if (K >= 64) {
+        const int64_t blocks = (rows + 3) / 4;  // four waves (rows) per 256-thread block
+        if (blocks > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "too many rows for one launch");
+        hipLaunchKernelGGL(rows_dot_wave_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, G, w, out, rows, K);
+    } else {
+        const int64_t blocks = (rows + 255) / 256;
+        if (blocks > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "too many rows for one launch");
+        hipLaunchKernelGGL(rows_dot_lane_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, G, w, out, rows, K);
+    }
+    LAUNCH_CHECK("rows_dot kernel");
+    return MM_OK;
+}
+
+// true when the lattice goes through the three-slab window (else whole, in the workspace)
+bool contract_windowed(const Shape &S, const DevState *ds) {
+    if (S.D == 1) return false;
+    int T = 64;
+    return choose_top(S, false, ds->max_lds, lds_maxt(S.D, false), &T) != 0;
+}
+
+size_t contract_ws_bytes(const Shape &S, const DevState *ds) {
+    if (!contract_windowed(S, ds)) return align256(size_t(S.N) * 16);
+    return 3 * align256(size_t(S.st[0]) * 16);
 }
 
 struct GlobalJob {
@@ -1528,6 +1576,57 @@ int mm_autoshape(const double *A, const double *b, const double *c, int64_t B, i
     hipLaunchKernelGGL(autoshape_schur_kernel, dim3(unsigned(B * M)), dim3(64), 0, s, a);
     hipLaunchKernelGGL(autoshape_recursion_kernel, dim3(unsigned((B * M + 255) / 256)), dim3(256), 0, s, a);
     LAUNCH_CHECK("autoshape kernels");
+    return MM_OK;
+}
+
+size_t mm_hermite_fwd_contract_workspace(const int64_t *shape, int D) {
+    Shape S;
+    if (make_shape(shape, D, &S)) return 0;
+    DevState *ds = nullptr;
+    if (device_state(&ds)) return 0;
+    return contract_ws_bytes(S, ds);
+}
+
+int mm_hermite_fwd_contract(const double *A, const double *b, const double *c, const int64_t *shape, int D, int n_head,
+                            const double *w, double *out, void *workspace, size_t ws_bytes, void *stream) {
+    Shape S;
+    int rc = make_shape(shape, D, &S);
+    if (rc) return rc;
+    if (n_head < 1 || n_head > D) return fail(MM_ERR_INVALID, "n_head=%d outside [1, %d]", n_head, D);
+    if (!A || !b || !c || !w || !out) return fail(MM_ERR_INVALID, "NULL array pointer");
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    const size_t need = contract_ws_bytes(S, ds);
+    if (!workspace || ws_bytes < need)
+        return fail(MM_ERR_WORKSPACE, "fused contraction workspace of %zu bytes needed, %zu given", need, ws_bytes);
+    int64_t K = 1;
+    for (int j = n_head; j < D; ++j) K *= shape[j];
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const double2 *w2 = reinterpret_cast<const double2 *>(w);
+    double2 *o2 = reinterpret_cast<double2 *>(out);
+    if (!contract_windowed(S, ds)) {
+        double2 *G = reinterpret_cast<double2 *>(workspace);
+        rc = vanilla_uniform(S, 1, A, b, c, nullptr, reinterpret_cast<double *>(G), nullptr, nullptr, nullptr, nullptr, 0, ds, s);
+        if (rc) return rc;
+        return launch_rows_dot(G, w2, o2, S.N / K, K, s);
+    }
+    // rows of one level-0 slab: n_head >= 1, so K divides stride[0]
+    const int64_t S0 = S.st[0], R = S0 / K;
+    const size_t slab_bytes = align256(size_t(S0) * 16);
+    char *base = reinterpret_cast<char *>(workspace);
+    double2 *slot[3] = {reinterpret_cast<double2 *>(base), reinterpret_cast<double2 *>(base + slab_bytes),
+                        reinterpret_cast<double2 *>(base + 2 * slab_bytes)};
+    rc = window_face(S, A, b, c, slot[0], ds, s);
+    if (rc) return rc;
+    for (int m = 0; m < S.s[0]; ++m) {
+        if (m >= 1) {
+            rc = window_slab0(S, A, b, slot[m % 3], slot[(m - 1) % 3], m >= 2 ? slot[(m - 2) % 3] : slot[(m - 1) % 3], m, ds, s);
+            if (rc) return rc;
+        }
+        rc = launch_rows_dot(slot[m % 3], w2, o2 + int64_t(m) * R, R, K, s);
+        if (rc) return rc;
+    }
     return MM_OK;
 }
 

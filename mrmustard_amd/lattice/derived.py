@@ -12,7 +12,12 @@ n + m variables (batch first) and c of shape ``batch + shape_derived_vars``.
   with n derived variables of size sum(shape_derived_vars) each, for m >= n.
 
 Both run the batched Hermite kernels and one batched product on the FP64
-matrix cores (``zgemm_batched_hip``) for the contraction with c.
+matrix cores (``zgemm_batched_hip``) for the contraction with c.  For large
+lattices ``fock_array_derived`` fuses the conversion with the contraction
+(``mm_hermite_fwd_contract``): the lattice is computed one slab of its first
+axis at a time in a three-slab window and each slab is contracted with c as
+soon as it exists, so the lattice on ``shape + shape_derived_vars`` is never
+stored whole.
 """
 
 from __future__ import annotations
@@ -21,8 +26,11 @@ import math as pymath
 
 import torch
 
+from .. import _lib
 from ..contraction import zgemm_batched_hip
-from .strategies import as_device_c128
+from .strategies import C128, _ptr, _shape_arr, _stream, _workspace, as_device_c128
+
+FUSE_MIN_AMPS = 1 << 20  # lattice entries per triple from which fock_array_derived fuses by default
 
 __all__ = ["decompose_ansatz_triple", "fock_array_derived"]
 
@@ -55,16 +63,48 @@ def _contract_derived(G, c, batch_shape, head_shape):
     return zgemm_batched_hip(G3, c3).reshape(tuple(batch_shape) + tuple(head_shape))
 
 
-def fock_array_derived(A, b, c, shape) -> torch.Tensor:
+def _fused_one(A1, b1, c1, full_shape, n):
+    """One triple through mm_hermite_fwd_contract: the lattice of (A1, b1, 1)
+    on ``full_shape`` contracted with c1 over its last axes as it is made."""
+    L = _lib.load()
+    s = _shape_arr(full_shape)
+    D = len(full_shape)
+    one = torch.ones((), dtype=C128, device=A1.device)
+    w = c1.reshape(-1).contiguous()
+    out = torch.empty(full_shape[:n], dtype=C128, device=A1.device)
+    ws = _workspace(L.mm_hermite_fwd_contract_workspace(s.ctypes.data, D))
+    rc = L.mm_hermite_fwd_contract(
+        _ptr(A1.contiguous()), _ptr(b1.contiguous()), _ptr(one), s.ctypes.data, D, n, _ptr(w), _ptr(out),
+        _ptr(ws), 0 if ws is None else ws.numel(), _stream(),
+    )
+    _lib.check(rc, "mm_hermite_fwd_contract")
+    return out
+
+
+def fock_array_derived(A, b, c, shape, fused: bool | None = None) -> torch.Tensor:
     """The Fock array of shape ``batch + shape`` of a triple whose c carries the
     derived variables (circuit_components.py:516-528): the lattice of (A, b, 1)
     on ``shape + shape_derived_vars``, contracted over the derived indices
-    with c.  ``shape`` has one entry per continuous variable."""
+    with c.  ``shape`` has one entry per continuous variable.  ``fused``
+    (default: from FUSE_MIN_AMPS lattice entries per triple) computes each
+    triple with the conversion fused with the contraction, never storing the
+    lattice whole; otherwise all triples go through the batched kernels and
+    one batched product."""
     A, b, c, batch_shape, shape_derived, n = _split(A, b, c)
     shape = tuple(int(s) for s in shape)
     if len(shape) != n:
         raise ValueError(f"shape {shape} has {len(shape)} entries for {n} continuous variables")
-    G = _lattice(A, b, batch_shape, shape + shape_derived)
+    full = shape + shape_derived
+    if fused is None:
+        fused = pymath.prod(full) >= FUSE_MIN_AMPS
+    if fused and n >= 1:
+        nb = pymath.prod(batch_shape) if batch_shape else 1
+        A3 = A.reshape(nb, *A.shape[-2:])
+        b3 = b.reshape(nb, b.shape[-1])
+        c3 = c.reshape(nb, *shape_derived)
+        outs = [_fused_one(A3[k], b3[k], c3[k], full, n) for k in range(nb)]
+        return torch.stack(outs).reshape(tuple(batch_shape) + shape)
+    G = _lattice(A, b, batch_shape, full)
     return _contract_derived(G, c, batch_shape, shape)
 
 

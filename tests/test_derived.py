@@ -114,3 +114,36 @@ def test_decompose_ansatz_triple(oracle_mod, n, shape_derived, shape):
     A3, b3, c3 = derived_triple(2, (3,), seed=5)
     out = decompose_ansatz_triple(A3, b3, c3)
     assert np.array_equal(out[0].cpu().numpy(), A3) and np.array_equal(out[2].cpu().numpy(), c3)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("n,shape_derived,shape", [(1, (3, 2), (6,)), (2, (2, 3), (4, 5)), (3, (2, 2), (3, 4, 2)), (2, (), (5, 6))],
+                         ids=lambda x: str(x))
+def test_fock_array_derived_fused_vs_oracle(oracle_mod, n, shape_derived, shape):
+    """The fused conversion + contraction on lattices small enough to be
+    computed whole, against the oracle's lattice and numpy's tensordot."""
+    from mrmustard_amd.lattice.derived import fock_array_derived
+
+    A, b, c = derived_triple(n, shape_derived, seed=7)
+    got = fock_array_derived(A, b, c, shape, fused=True).cpu().numpy()
+    assert got.shape == shape
+    assert relerr(got, oracle_fock_derived(oracle_mod, A, b, c, shape)) < TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("n,shape_derived,shape", [(1, (30, 40), (25,)), (2, (20, 20), (30, 30)), (2, (100,), (40, 50)),
+                                                   (3, (3, 4), (12, 14, 16))], ids=lambda x: str(x))
+def test_fock_array_derived_fused_window(n, shape_derived, shape):
+    """Lattices too large for LDS take the three-slab window: the same array
+    as the unfused path (the whole lattice from the bit-exact vanilla kernel,
+    then a product on the matrix cores) to rounding, for short rows (one lane
+    each) and long ones (one wave each), and a batch of two."""
+    from mrmustard_amd.lattice.derived import fock_array_derived
+
+    A, b, c = derived_triple(n, shape_derived, seed=8, batch=(2,))
+    ref = fock_array_derived(A, b, c, shape, fused=False).cpu().numpy()
+    got = fock_array_derived(A, b, c, shape, fused=True).cpu().numpy()
+    assert got.shape == (2,) + shape
+    assert relerr(got, ref) < TOL
