@@ -398,6 +398,30 @@ def main():
                     "call runs the forward again, then two reverse launches per weight",
         }
 
+        # ---- derived variables: the conversion fused with the contraction with c
+        log("derived variables: (40,40) kept, (25,25) derived, fused vs lattice + product")
+        from mrmustard_amd.lattice.derived import fock_array_derived
+
+        rng_d = np.random.default_rng(11)
+        Xd = rng_d.standard_normal((4, 4)) + 1j * rng_d.standard_normal((4, 4))
+        Adv = torch.as_tensor(0.15 * (Xd + Xd.T), device=dev)
+        bdv = torch.as_tensor(0.3 * (rng_d.standard_normal(4) + 1j * rng_d.standard_normal(4)), device=dev)
+        cdv = torch.as_tensor(rng_d.standard_normal((25, 25)) + 1j * rng_d.standard_normal((25, 25)), device=dev)
+        t_fu = timed(lambda: fock_array_derived(Adv, bdv, cdv, (40, 40), fused=True), 10, 2)
+        t_un = timed(lambda: fock_array_derived(Adv, bdv, cdv, (40, 40), fused=False), 10, 2)
+        r_fu = fock_array_derived(Adv, bdv, cdv, (40, 40), fused=True)
+        r_un = fock_array_derived(Adv, bdv, cdv, (40, 40), fused=False)
+        extra["derived_40x40_25x25"] = {
+            "lattice_amplitudes": 40 * 40 * 25 * 25,
+            "fused_ms_per_call": t_fu * 1e3,
+            "unfused_ms_per_call": t_un * 1e3,
+            "fused_resident_bytes": 3 * 40 * 25 * 25 * 16,
+            "unfused_resident_bytes": 40 * 40 * 25 * 25 * 16,
+            "rel_err_fused_vs_unfused": float((r_fu - r_un).abs().max() / r_un.abs().max()),
+            "note": "one triple; fused = mm_hermite_fwd_contract (three-slab window, each slab contracted as made); "
+                    "unfused = whole lattice, then a product on the matrix cores",
+        }
+
     cpu = None
     parity = None
     if not args.no_cpu and not args.quick:
