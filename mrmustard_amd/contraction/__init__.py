@@ -34,7 +34,8 @@ __all__ = ["contract", "dm_operator_product", "einsum", "zgemm_batched_hip", "zg
 
 def zgemm_hip(A, B, out: torch.Tensor | None = None) -> torch.Tensor:
     """C = A @ B for complex128 matrices on the GPU (row-major, any strides
-    whose last dimension is contiguous are copied to contiguous first)."""
+    whose last dimension is contiguous are copied to contiguous first).
+    Differentiable when an operand requires grad (then without ``out``)."""
     A = as_device_c128(A)
     B = as_device_c128(B)
     if A.dim() != 2 or B.dim() != 2:
@@ -43,6 +44,10 @@ def zgemm_hip(A, B, out: torch.Tensor | None = None) -> torch.Tensor:
     K2, N = B.shape
     if K != K2:
         raise ValueError(f"inner dimensions differ: {tuple(A.shape)} @ {tuple(B.shape)}")
+    if torch.is_grad_enabled() and (A.requires_grad or B.requires_grad):
+        if out is not None:
+            raise ValueError("out is not supported when gradients are tracked")
+        return zgemm_batched_hip(A[None], B[None])[0]
     if out is None:
         out = torch.empty((M, N), dtype=torch.complex128, device=A.device)
     elif out.shape != (M, N) or out.dtype != torch.complex128 or not out.is_contiguous() or out.device != A.device:
@@ -78,9 +83,37 @@ def dm_operator_product(P, rho, out: torch.Tensor | None = None) -> torch.Tensor
     return C.reshape(rho.shape)
 
 
+class _ZgemmBatched(torch.autograd.Function):
+    """C = A @ B batched, differentiable as the JAX backend's einsum is
+    (backend_jax.py einsum under jax.grad): with torch's convention for complex
+    tensors, dL/dA = dL/dC @ B^H and dL/dB = A^H @ dL/dC, summed over the
+    batch for an operand that was broadcast.  Both are products on the same
+    matrix-core kernel."""
+
+    @staticmethod
+    def forward(ctx, A, B):
+        ctx.save_for_backward(A, B)
+        return _zgemm_batched(A, B)
+
+    @staticmethod
+    def backward(ctx, gC):
+        A, B = ctx.saved_tensors
+        gA = gB = None
+        if ctx.needs_input_grad[0]:
+            gA = _zgemm_batched(gC, B.transpose(1, 2).conj())
+            if A.shape[0] == 1 and gA.shape[0] != 1:
+                gA = gA.sum(0, keepdim=True)
+        if ctx.needs_input_grad[1]:
+            gB = _zgemm_batched(A.transpose(1, 2).conj(), gC)
+            if B.shape[0] == 1 and gB.shape[0] != 1:
+                gB = gB.sum(0, keepdim=True)
+        return gA, gB
+
+
 def zgemm_batched_hip(A, B) -> torch.Tensor:
     """C[b] = A[b] @ B[b] for (batch, M, K) and (batch, K, N) complex128
-    tensors; an operand with batch 1 is broadcast."""
+    tensors; an operand with batch 1 is broadcast.  Differentiable when an
+    operand requires grad."""
     A = as_device_c128(A)
     B = as_device_c128(B)
     if A.dim() != 3 or B.dim() != 3:
@@ -91,6 +124,18 @@ def zgemm_batched_hip(A, B) -> torch.Tensor:
         raise ValueError(f"inner dimensions differ: {tuple(A.shape)} @ {tuple(B.shape)}")
     if ba != bb and 1 not in (ba, bb):
         raise ValueError(f"batch sizes differ: {ba} and {bb}")
+    if torch.is_grad_enabled() and (A.requires_grad or B.requires_grad):
+        return _ZgemmBatched.apply(A, B)
+    return _zgemm_batched(A, B)
+
+
+def _zgemm_batched(A, B) -> torch.Tensor:
+    """The kernel call of zgemm_batched_hip (shapes already checked); lazy
+    conjugate views are materialised before their pointers are taken."""
+    A = as_device_c128(A)
+    B = as_device_c128(B)
+    ba, M, K = A.shape
+    bb, _, N = B.shape
     nb = max(ba, bb)
     out = torch.empty((nb, M, N), dtype=torch.complex128, device=A.device)
     if K == 0 or out.numel() == 0:

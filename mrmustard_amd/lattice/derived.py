@@ -28,6 +28,7 @@ import torch
 
 from .. import _lib
 from ..contraction import zgemm_batched_hip
+from ..settings import settings
 from .strategies import C128, _ptr, _shape_arr, _stream, _workspace, as_device_c128
 
 FUSE_MIN_AMPS = 1 << 20  # lattice entries per triple from which fock_array_derived fuses by default
@@ -88,8 +89,9 @@ def fock_array_derived(A, b, c, shape, fused: bool | None = None) -> torch.Tenso
     with c.  ``shape`` has one entry per continuous variable.  ``fused``
     (default: from FUSE_MIN_AMPS lattice entries per triple) computes each
     triple with the conversion fused with the contraction, never storing the
-    lattice whole; otherwise all triples go through the batched kernels and
-    one batched product."""
+    lattice whole; otherwise (and always with gradients tracked or
+    settings.STABLE_FOCK_CONVERSION) all triples go through the batched kernels
+    and one batched product."""
     A, b, c, batch_shape, shape_derived, n = _split(A, b, c)
     shape = tuple(int(s) for s in shape)
     if len(shape) != n:
@@ -97,7 +99,10 @@ def fock_array_derived(A, b, c, shape, fused: bool | None = None) -> torch.Tenso
     full = shape + shape_derived
     if fused is None:
         fused = pymath.prod(full) >= FUSE_MIN_AMPS
-    if fused and n >= 1:
+    # the fused kernel computes the vanilla lattice and has no backward: the
+    # stable conversion and tracked gradients go through the lattice + product
+    tracked = torch.is_grad_enabled() and any(t.requires_grad for t in (A, b, c))
+    if fused and n >= 1 and not tracked and not settings.STABLE_FOCK_CONVERSION:
         nb = pymath.prod(batch_shape) if batch_shape else 1
         A3 = A.reshape(nb, *A.shape[-2:])
         b3 = b.reshape(nb, b.shape[-1])

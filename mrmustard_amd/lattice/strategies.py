@@ -63,6 +63,8 @@ def as_device_c128(x) -> torch.Tensor:
             t = t.to(dev)
         if t.is_conj():
             t = t.resolve_conj()
+        if t.is_neg():
+            t = t.resolve_neg()
         return t.contiguous()
     return torch.as_tensor(np.asarray(x, dtype=np.complex128), device=dev).contiguous()
 

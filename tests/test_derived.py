@@ -147,3 +147,37 @@ def test_fock_array_derived_fused_window(n, shape_derived, shape):
     got = fock_array_derived(A, b, c, shape, fused=True).cpu().numpy()
     assert got.shape == (2,) + shape
     assert relerr(got, ref) < TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("fused", [False, True])
+def test_fock_array_derived_gradients(fused):
+    """With A, b, c tracked, the derived-variables array is differentiable (a
+    fused request falls back to the lattice + product): grads equal those of
+    the same composition with torch's tensordot as the contraction."""
+    import torch
+
+    from mrmustard_amd import math as mm
+    from mrmustard_amd.lattice.derived import fock_array_derived
+
+    A, b, c = derived_triple(2, (3, 4), seed=9)
+    shape = (5, 6)
+    rng = np.random.default_rng(3)
+    w = torch.tensor(rng.standard_normal(shape) + 1j * rng.standard_normal(shape), device="cuda")
+
+    def grads(contract):
+        At = torch.tensor(A, device="cuda", requires_grad=True)
+        bt = torch.tensor(b, device="cuda", requires_grad=True)
+        ct = torch.tensor(c, device="cuda", requires_grad=True)
+        torch.real(torch.sum(w * contract(At, bt, ct))).backward()
+        return At.grad.cpu().numpy(), bt.grad.cpu().numpy(), ct.grad.cpu().numpy()
+
+    def ref(At, bt, ct):
+        G = mm.hermite_renormalized(At, bt, torch.ones((), dtype=torch.complex128, device="cuda"), shape=shape + (3, 4))
+        return torch.tensordot(G, ct, dims=([2, 3], [0, 1]))
+
+    got = grads(lambda At, bt, ct: fock_array_derived(At, bt, ct, shape, fused=fused))
+    exp = grads(ref)
+    for g, e in zip(got, exp):
+        assert relerr(g, e) < TOL
