@@ -91,6 +91,42 @@ def test_stable_exact(st, oracle_mod, shape):
     assert np.array_equal(G, ref), rel(G, ref)
 
 
+@pytest.mark.parametrize("shape", [(600, 500), (70, 70, 60), (40, 40, 40, 5), (12, 10, 10, 10, 10, 3)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_stable_exact_large(st, oracle_mod, shape):
+    """Lattices above 2^18 entries take one launch per index-sum wavefront
+    instead of one workgroup per triple: bit-exact."""
+    rng = np.random.default_rng(sum(shape) + 5)
+    A, b, c = triple(rng, len(shape), sym=False)
+    print(f"stable large shape={shape}", flush=True)
+    G = np_(st.stable_hip(shape, A, b, c))
+    ref = oracle_mod.stable(shape, A, b, c)
+    assert np.array_equal(G, ref), rel(G, ref)
+
+
+def test_stable_batch_large(st, oracle_mod):
+    """A batch of two large lattices: each triple's wavefront sweep in turn."""
+    rng = np.random.default_rng(77)
+    shape = (70, 70, 60)
+    trip = [triple(rng, 3) for _ in range(2)]
+    A = np.stack([t[0] for t in trip])
+    b = np.stack([t[1] for t in trip])
+    c = np.array([t[2] for t in trip])
+    print(f"stable batch large shape={shape}", flush=True)
+    Gs = np_(st.vanilla_batch_hip(shape, A, b, c, stable=True))
+    assert np.array_equal(Gs, oracle_mod.vanilla_batch(shape, A, b, c, stable=True))
+
+
+def test_stable_lossy_dm_40x4(st, oracle_mod):
+    from mrmustard_amd import workloads as W
+
+    (A, b, c), shape = W.lossy_dm_40x4()
+    print(f"stable lossy DM shape={shape}", flush=True)
+    G = np_(st.stable_hip(shape, A, b, c))
+    ref = oracle_mod.stable(shape, A, b, c)
+    assert np.array_equal(G, ref), rel(G, ref)
+
+
 def test_stable_large_displacement(st):
     """test_lattice_functions.py:141-149 with a reduced size: the stable
     recurrence stays bounded for a large displacement and matches <m|D|n>."""
