@@ -213,15 +213,93 @@ def contract(array1, array2, idx1, idx2, idx_out) -> torch.Tensor:
     return C.permute(*[order.index(lab) for lab in idx_out]).contiguous()
 
 
-def einsum(string: str, *tensors) -> torch.Tensor:
-    """``math.einsum`` (backend_manager.py:526-548) for two operands whose
-    labels each appear once per operand, e.g. ``"abc,cd->abd"``."""
-    if len(tensors) != 2:
-        raise NotImplementedError(f"einsum on the GPU takes two operands, got {len(tensors)}")
-    if "->" not in string:
-        raise NotImplementedError("einsum on the GPU needs an explicit output ('->')")
-    ins, out = string.replace(" ", "").split("->")
+def einsum_labels(string: str, ndims) -> tuple[list[list[str]], list[str]]:
+    """Per-operand label lists and the output labels of an einsum string, with
+    numpy's rules: ``...`` stands for an operand's leading axes not named
+    explicitly (right-aligned across operands, given labels of its own), and
+    without ``->`` the output is the labels that appear once, sorted, after the
+    ellipsis axes."""
+    string = string.replace(" ", "")
+    ins, out = string.split("->") if "->" in string else (string, None)
     parts = ins.split(",")
-    if len(parts) != 2 or "." in string:
-        raise ValueError(f"bad einsum string {string!r} for two operands")
-    return contract(tensors[0], tensors[1], list(parts[0]), list(parts[1]), list(out))
+    if len(parts) != len(ndims):
+        raise ValueError(f"einsum string {string!r} names {len(parts)} operands, got {len(ndims)}")
+    n_ell = 0
+    for part, nd in zip(parts, ndims):
+        if "..." in part:
+            n_ell = max(n_ell, nd - len(part.replace("...", "")))
+    ell = [f"...{k}" for k in range(n_ell)]  # labels of the ellipsis axes, right-aligned
+
+    def expand(part, nd):
+        if "..." not in part:
+            if len(part) != nd:
+                raise ValueError(f"operand with {nd} axes labelled {part!r}")
+            return list(part)
+        head, tail = part.split("...")
+        k = nd - len(head) - len(tail)
+        if k < 0:
+            raise ValueError(f"operand with {nd} axes labelled {part!r}")
+        return list(head) + ell[n_ell - k:] + list(tail)
+
+    labels = [expand(part, nd) for part, nd in zip(parts, ndims)]
+    if out is None:
+        counts = {}
+        for labs in labels:
+            for lab in labs:
+                counts[lab] = counts.get(lab, 0) + 1
+        outl = ell + sorted(lab for lab, n in counts.items() if n == 1 and not lab.startswith("..."))
+    else:
+        outl = list(out.split("...")[0]) + (ell if "..." in out else []) + (list(out.split("...")[1]) if "..." in out else [])
+    return labels, outl
+
+
+def _take_diagonals(t, labs):
+    """Collapse a label repeated within one operand to its diagonal."""
+    labs = list(labs)
+    while len(set(labs)) != len(labs):
+        lab = next(x for x in labs if labs.count(x) > 1)
+        i = labs.index(lab)
+        j = labs.index(lab, i + 1)
+        t = torch.diagonal(t, dim1=i, dim2=j)  # the diagonal axis goes last
+        labs = [x for k, x in enumerate(labs) if k not in (i, j)] + [lab]
+    return t, labs
+
+
+def einsum(string: str, *tensors) -> torch.Tensor:
+    """``math.einsum`` (backend_manager.py:526-548) on the GPU, with numpy's
+    string rules (``...``, implicit output, a label repeated within an operand
+    for its diagonal).  Operands are contracted pairwise, left to right, each
+    pair as one batched product on the matrix cores (``contract``); an
+    intermediate keeps the labels the output or a later operand still needs.
+    Shared axes are expected to have equal sizes (``contract`` cuts unequal
+    ones to the smaller, as ArrayAnsatz.contract does)."""
+    if not tensors:
+        raise ValueError("einsum needs at least one operand")
+    ts = [as_device_c128(t) for t in tensors]
+    labels, outl = einsum_labels(string, [t.dim() for t in ts])
+    if len(set(outl)) != len(outl):
+        raise ValueError(f"output labels of {string!r} repeat")
+    every = set().union(*labels)
+    if not set(outl).issubset(every):
+        raise ValueError("Output labels must be present in input labels.")
+    sizes = {}
+    for t, labs in zip(ts, labels):
+        for lab, n in zip(labs, t.shape):
+            if sizes.setdefault(lab, int(n)) != int(n):
+                raise ValueError(f"einsum {string!r}: axis {lab!r} has sizes {sizes[lab]} and {int(n)}")
+    pairs = [_take_diagonals(t, labs) for t, labs in zip(ts, labels)]
+    res, rl = pairs[0]
+    for k in range(1, len(pairs)):
+        t, tl = pairs[k]
+        later = set(outl).union(*[set(lb) for _, lb in pairs[k + 1:]])
+        keep = [lab for lab in rl + tl if lab in later]
+        keep = list(dict.fromkeys(keep))
+        res, rl = contract(res, t, rl, tl, keep), keep
+    # one operand left: sum what the output does not name, then order the axes
+    lone = [i for i, lab in enumerate(rl) if lab not in outl]
+    if lone:
+        res = res.sum(dim=lone)
+        rl = [lab for lab in rl if lab in outl]
+    if not outl:
+        return res.reshape(())
+    return res.permute(*[rl.index(lab) for lab in outl]).contiguous()

@@ -174,10 +174,11 @@ class BackendManager:
         return self._apply("hermite_renormalized_1leftoverMode", (A, b, c, output_cutoff, pnr_cutoffs, reorderedAB))
 
     def einsum(self, string: str, *tensors, optimize="greedy", backend=None):
-        r"""The Einstein summation of two Fock-basis arrays on the FP64 matrix
-        cores: the contraction ``ArrayAnsatz.contract`` hands to ``math.einsum``
-        (backend_manager.py:526-548).  ``optimize`` and ``backend`` are accepted
-        for the reference's signature; a pair has one contraction order."""
+        r"""Einstein summation of Fock-basis arrays on the FP64 matrix cores
+        (backend_manager.py:526-548), as ``ArrayAnsatz.contract`` and the
+        derived-variables branch of ``fock_array`` call it: numpy's string
+        rules, operands contracted pairwise left to right.  ``optimize`` and
+        ``backend`` are accepted for the reference's signature."""
         from ..contraction import einsum as _einsum
 
         return _einsum(string, *tensors)

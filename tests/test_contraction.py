@@ -93,9 +93,10 @@ def test_contract_errors():
     with pytest.raises(ValueError, match="Output labels"):
         contract(a, b, [0, 1], [1, 2], [3])
     with pytest.raises(NotImplementedError):
-        einsum("ii,ij->j", a, b)
-    with pytest.raises(NotImplementedError):
-        einsum("ij,jk,kl->il", a, b, a)
+        contract(a, b, [0, 0], [0, 1], [1])  # a diagonal: einsum handles it, contract does not
+    # what einsum now takes: a diagonal and three operands
+    assert relerr(einsum("ii,ij->j", a, b).cpu().numpy(), np.einsum("ii,ij->j", a, b)) < TOL
+    assert relerr(einsum("ij,jk,kl->il", a, b, a).cpu().numpy(), a @ b @ a) < TOL
 
 
 @pytest.mark.gpu
@@ -141,3 +142,70 @@ def test_contraction_gradients_match_torch_autograd():
     ra = torch.tensor(M1, requires_grad=True)
     torch.real(torch.sum(ra @ torch.tensor(M2))).backward()
     assert np.max(np.abs(xa.grad.cpu().numpy() - ra.grad.numpy())) < 1e-12 * np.max(np.abs(ra.grad.numpy()))
+
+
+def test_einsum_labels():
+    """numpy's einsum string rules, on the CPU: ellipsis axes right-aligned with
+    labels of their own, implicit output = labels seen once, sorted."""
+    from mrmustard_amd.contraction import einsum_labels
+
+    labs, out = einsum_labels("...abc,...c->...ab", [5, 3])
+    assert labs == [["...0", "...1", "a", "b", "c"], ["...0", "...1", "c"]] and out == ["...0", "...1", "a", "b"]
+    assert einsum_labels("ij,jk", [2, 2]) == ([["i", "j"], ["j", "k"]], ["i", "k"])
+    assert einsum_labels("ba,c", [2, 1]) == ([["b", "a"], ["c"]], ["a", "b", "c"])
+    labs, out = einsum_labels("...ab,b->...a", [4, 1])
+    assert labs == [["...0", "...1", "a", "b"], ["b"]] and out == ["...0", "...1", "a"]
+    with pytest.raises(ValueError):
+        einsum_labels("ab,bc->ac", [2])
+    with pytest.raises(ValueError):
+        einsum_labels("abc->a", [2])
+
+
+EINSUM_CASES = [
+    ("abc,cd->abd", [(3, 4, 5), (5, 6)]),
+    ("ij,jk", [(4, 5), (5, 3)]),
+    ("...abc,...c->...ab", [(2, 3, 4, 5, 6), (2, 3, 6)]),
+    ("ii->i", [(5, 5)]),
+    ("ab->", [(4, 7)]),
+    ("ba", [(3, 4)]),
+    ("ij,jk,kl->il", [(3, 4), (4, 5), (5, 2)]),
+    ("abc,bd,ce->ade", [(2, 3, 4), (3, 5), (4, 6)]),
+    ("aab,b->a", [(4, 4, 3), (3,)]),
+    ("i,i->", [(7,), (7,)]),
+    ("ab,cd->abcd", [(2, 3), (4, 5)]),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+@pytest.mark.parametrize("spec,shapes", EINSUM_CASES, ids=lambda x: str(x))
+def test_einsum_general_vs_torch(spec, shapes):
+    import torch
+
+    from mrmustard_amd.contraction import einsum
+
+    rng = np.random.default_rng(len(spec))
+    arrs = [rng.standard_normal(s) + 1j * rng.standard_normal(s) for s in shapes]
+    got = einsum(spec, *[torch.tensor(a, device="cuda") for a in arrs]).cpu().numpy()
+    ref = torch.einsum(spec, *[torch.tensor(a) for a in arrs]).numpy()
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+def test_einsum_three_operands_gradient_and_errors():
+    import torch
+
+    from mrmustard_amd.contraction import einsum
+
+    rng = np.random.default_rng(4)
+    arrs = [rng.standard_normal(s) + 1j * rng.standard_normal(s) for s in [(3, 4), (4, 5), (5, 2)]]
+    xs = [torch.tensor(a, device="cuda", requires_grad=True) for a in arrs]
+    torch.real(torch.sum(einsum("ij,jk,kl->il", *xs))).backward()
+    rs = [torch.tensor(a, requires_grad=True) for a in arrs]
+    torch.real(torch.sum(torch.einsum("ij,jk,kl->il", *rs))).backward()
+    for x, r in zip(xs, rs):
+        assert np.max(np.abs(x.grad.cpu().numpy() - r.grad.numpy())) < 1e-12 * np.max(np.abs(r.grad.numpy()))
+    with pytest.raises(ValueError, match="sizes"):
+        einsum("ij,jk->ik", torch.zeros((2, 3), device="cuda"), torch.zeros((4, 2), device="cuda"))
