@@ -268,11 +268,12 @@ def _take_diagonals(t, labs):
 def einsum(string: str, *tensors) -> torch.Tensor:
     """``math.einsum`` (backend_manager.py:526-548) on the GPU, with numpy's
     string rules (``...``, implicit output, a label repeated within an operand
-    for its diagonal).  Operands are contracted pairwise, left to right, each
-    pair as one batched product on the matrix cores (``contract``); an
-    intermediate keeps the labels the output or a later operand still needs.
-    Shared axes are expected to have equal sizes (``contract`` cuts unequal
-    ones to the smaller, as ArrayAnsatz.contract does)."""
+    for its diagonal, size-1 ellipsis axes broadcast).  Operands are contracted
+    pairwise, each pair as one batched product on the matrix cores
+    (``contract``), in a greedy order: the pair whose result is smallest first,
+    as opt_einsum's "greedy" prefers; an intermediate keeps the labels the
+    output or another operand still needs.  Other shared axes must have equal
+    sizes (ValueError otherwise, as numpy)."""
     if not tensors:
         raise ValueError("einsum needs at least one operand")
     ts = [as_device_c128(t) for t in tensors]
@@ -285,16 +286,35 @@ def einsum(string: str, *tensors) -> torch.Tensor:
     sizes = {}
     for t, labs in zip(ts, labels):
         for lab, n in zip(labs, t.shape):
-            if sizes.setdefault(lab, int(n)) != int(n):
-                raise ValueError(f"einsum {string!r}: axis {lab!r} has sizes {sizes[lab]} and {int(n)}")
+            n = int(n)
+            m = sizes.setdefault(lab, n)
+            if m != n:
+                if not (lab.startswith("...") and 1 in (m, n)):
+                    raise ValueError(f"einsum {string!r}: axis {lab!r} has sizes {m} and {n}")
+                sizes[lab] = max(m, n)
+    # size-1 ellipsis axes broadcast against the others, as numpy's do
+    ts = [t.expand(*[sizes[lab] for lab in labs]) if any(int(n) != sizes[lab] for lab, n in zip(labs, t.shape)) else t
+          for t, labs in zip(ts, labels)]
     pairs = [_take_diagonals(t, labs) for t, labs in zip(ts, labels)]
+    # greedy order: contract the pair whose result is smallest (ties: leftmost)
+    while len(pairs) > 1:
+        best = None
+        for i in range(len(pairs)):
+            for j in range(i + 1, len(pairs)):
+                rest = [lb for k, (_, lb) in enumerate(pairs) if k not in (i, j)]
+                later = set(outl).union(*[set(lb) for lb in rest])
+                keep = list(dict.fromkeys(lab for lab in pairs[i][1] + pairs[j][1] if lab in later))
+                n_out = 1
+                for lab in keep:
+                    n_out *= sizes[lab]
+                if best is None or n_out < best[0]:
+                    best = (n_out, i, j, keep)
+        _, i, j, keep = best
+        (ti, li), (tj, lj) = pairs[i], pairs[j]
+        merged = (contract(ti, tj, li, lj, keep), keep)
+        pairs = [p for k, p in enumerate(pairs) if k not in (i, j)]
+        pairs.insert(i, merged)
     res, rl = pairs[0]
-    for k in range(1, len(pairs)):
-        t, tl = pairs[k]
-        later = set(outl).union(*[set(lb) for _, lb in pairs[k + 1:]])
-        keep = [lab for lab in rl + tl if lab in later]
-        keep = list(dict.fromkeys(keep))
-        res, rl = contract(res, t, rl, tl, keep), keep
     # one operand left: sum what the output does not name, then order the axes
     lone = [i for i, lab in enumerate(rl) if lab not in outl]
     if lone:

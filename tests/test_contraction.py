@@ -173,6 +173,9 @@ EINSUM_CASES = [
     ("aab,b->a", [(4, 4, 3), (3,)]),
     ("i,i->", [(7,), (7,)]),
     ("ab,cd->abcd", [(2, 3), (4, 5)]),
+    ("...ab,...b->...a", [(2, 1, 3, 4), (1, 5, 4)]),
+    ("ab,cd,bc->ad", [(3, 40), (50, 2), (40, 50)]),
+    ("i,ij,jk,k->", [(6,), (6, 7), (7, 8), (8,)]),
 ]
 
 
