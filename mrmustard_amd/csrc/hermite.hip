@@ -998,8 +998,7 @@ size_t lo_vjp_bytes(const DiagGeom &g, int Mt, int64_t C, size_t fwd_bytes, int 
 }
 
 template <int MT, bool REORDER>
-void launch_lo_vjp(LeftVjpArgs &v, const std::vector<int64_t> &counts, int nblk, int64_t nmirror, double2 *dA, double2 *dB,
-                   hipStream_t s) {
+void launch_lo_vjp(LeftVjpArgs &v, const std::vector<int64_t> &counts, int nblk, double2 *dA, double2 *dB, hipStream_t s) {
     constexpr int PB = DG_THREADS / (2 * MT);
     constexpr int GL = 2 * (MT + 1);
     const int64_t C = v.a.C, P = v.a.t.P;
@@ -1007,9 +1006,8 @@ void launch_lo_vjp(LeftVjpArgs &v, const std::vector<int64_t> &counts, int nblk,
         const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256));
         hipLaunchKernelGGL(zero_c128_span_kernel, dim3(unsigned(grid)), dim3(256), 0, s, p, n);
     };
-    zero(v.adD, nmirror);
-    zero(v.adP, MT * P * C);
-    zero(v.part, int64_t(nblk + 1) * GL * (GL + 1));
+    // the mirrors, adP and the partial rows are one contiguous span of the workspace
+    zero(v.adD, (v.part + int64_t(nblk + 1) * GL * (GL + 1)) - v.adD);
     std::vector<int64_t> starts(counts.size(), 0);
     for (size_t w = 1; w < counts.size(); ++w) starts[w] = starts[w - 1] + counts[w - 1];
     for (int w = v.a.t.S; w >= 0; --w) {
@@ -1022,7 +1020,9 @@ void launch_lo_vjp(LeftVjpArgs &v, const std::vector<int64_t> &counts, int nblk,
         hipLaunchKernelGGL((lo_off_vjp_kernel<MT, REORDER>), dim3(nw), dim3(DG_THREADS), 0, s, v, counts[w], starts[w], iters);
         hipLaunchKernelGGL((lo_diag_vjp_kernel<MT, REORDER>), dim3(nw), dim3(DG_THREADS), 0, s, v, counts[w], starts[w], iters);
     }
-    hipLaunchKernelGGL((lo_init_vjp_kernel<MT, REORDER>), dim3(1), dim3(1024), 0, s, v, nblk);
+    int nt = 64;
+    while (nt < v.a.c0) nt <<= 1;  // c0 <= 1024
+    hipLaunchKernelGGL((lo_init_vjp_kernel<MT, REORDER>), dim3(1), dim3(nt), 0, s, v, nblk);
     hipLaunchKernelGGL((diag_vjp_finalize<MT + 1, REORDER>), dim3(GL * (GL + 1)), dim3(256), 0, s, v.part, nblk + 1, dA, dB);
 }
 
@@ -1701,8 +1701,8 @@ int mm_hermite_1leftover_vjp(const double *A, const double *b, const double *c, 
     double2 *dA2 = reinterpret_cast<double2 *>(dA), *dB2 = reinterpret_cast<double2 *>(db);
     switch (Mt * 2 + (reordered ? 1 : 0)) {
 #define CASE(MM)                                                                        \
-    case 2 * MM: launch_lo_vjp<MM, false>(v, counts, nblk, nmirror, dA2, dB2, s); break; \
-    case 2 * MM + 1: launch_lo_vjp<MM, true>(v, counts, nblk, nmirror, dA2, dB2, s); break;
+    case 2 * MM: launch_lo_vjp<MM, false>(v, counts, nblk, dA2, dB2, s); break; \
+    case 2 * MM + 1: launch_lo_vjp<MM, true>(v, counts, nblk, dA2, dB2, s); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
         default:

@@ -289,7 +289,8 @@ __global__ __launch_bounds__(DG_THREADS) void lo_diag_vjp_kernel(LeftVjpArgs v, 
 // The zero-detection block in reverse (singleLeftoverMode_amps.py:330-341):
 // columns c0-1 .. 1 one step each, thread m owning row m of the adjoint
 // (inbox slot 0 of position 0, entry (m, n) at m * c0 + n), then column 0 by
-// thread 0.  Its dA rows 0-1 / dB go to partial row nblk.  c0 <= 1024.
+// thread 0.  Its dA rows 0-1 / dB go to partial row nblk.  blockDim.x is a
+// power of two >= max(64, c0), at most 1024.
 template <int MT, bool REORDER>
 __global__ __launch_bounds__(1024) void lo_init_vjp_kernel(LeftVjpArgs v, int nblk) {
     constexpr int L = MT + 1, GL = 2 * L;
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(1024) void lo_init_vjp_kernel(LeftVjpArgs v, int nb
     const double2 z = make_double2(0.0, 0.0);
     double2 dB1 = z, dA10 = z, dA11 = z;
     sw[m] = z;
-    if (m == 0) sw[1024] = z;
+    if (m == 0) sw[blockDim.x] = z;
     __syncthreads();
     for (int k = c0 - 1; k >= 1; --k) {
         // column k = n + 1 from columns n and n - 1
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(1024) void lo_init_vjp_kernel(LeftVjpArgs v, int nb
     for (int q = 0; q < 3; ++q) {
         red[m] = acc[q];
         __syncthreads();
-        for (int h = 512; h > 0; h >>= 1) {
+        for (int h = int(blockDim.x) >> 1; h > 0; h >>= 1) {
             if (m < h) red[m] = cadd(red[m], red[m + h]);
             __syncthreads();
         }
