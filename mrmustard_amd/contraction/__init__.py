@@ -153,8 +153,9 @@ def contract(array1, array2, idx1, idx2, idx_out) -> torch.Tensor:
     """``ArrayAnsatz.contract`` (array_ansatz.py:159-225) of two arrays on the
     GPU: labels (str or int) name the axes of each array; shared labels whose
     sizes differ are cut to the smaller size, as the reference does; the result
-    has the axes ``idx_out``.  Each label may appear once per operand; a label
-    of one operand that is not in the output is summed over."""
+    has the axes ``idx_out``.  A label repeated within an operand takes that
+    operand's diagonal (numpy's einsum rule); a label of one operand that is
+    not in the output is summed over."""
     a = as_device_c128(array1)
     b = as_device_c128(array2)
     idx1, idx2, idx_out = list(idx1), list(idx2), list(idx_out)
@@ -164,19 +165,31 @@ def contract(array1, array2, idx1, idx2, idx_out) -> torch.Tensor:
         raise ValueError(f"expected len(idx2)={b.dim()} got {len(idx2)}")
     if not set(idx_out).issubset(set(idx1) | set(idx2)):
         raise ValueError("Output labels must be present in input labels.")
-    for idx in (idx1, idx2, idx_out):
-        if len(set(idx)) != len(idx):
-            raise NotImplementedError("a label repeated within one operand (a trace or diagonal) is not supported")
+    if len(set(idx_out)) != len(idx_out):
+        raise ValueError(f"output labels {idx_out} repeat")
     shared = set(idx1) & set(idx2)
     sl1 = [slice(None)] * a.dim()
     sl2 = [slice(None)] * b.dim()
-    for lab in shared:
+    for lab in shared:  # the first axis of a label is cut, as the reference does
         p1, p2 = idx1.index(lab), idx2.index(lab)
         d = min(a.shape[p1], b.shape[p2])
         sl1[p1] = slice(0, d)
         sl2[p2] = slice(0, d)
     a = a[tuple(sl1)]
     b = b[tuple(sl2)]
+    # a label repeated within an operand takes that operand's diagonal first
+    for k, (t, idx) in enumerate(((a, idx1), (b, idx2))):
+        if len(set(idx)) == len(idx):
+            continue
+        for lab in set(idx):
+            dims = {int(n) for x, n in zip(idx, t.shape) if x == lab}
+            if len(dims) > 1:
+                raise ValueError(f"the repeated axis {lab!r} of operand {k + 1} has sizes {sorted(dims)}")
+        t, idx = _take_diagonals(t, idx)
+        if k == 0:
+            a, idx1 = t, idx
+        else:
+            b, idx2 = t, idx
     # a label of one operand only, absent from the output, is summed first
     for arr, idx in ((0, idx1), (1, idx2)):
         lone = [lab for lab in idx if lab not in shared and lab not in idx_out]
@@ -200,9 +213,9 @@ def contract(array1, array2, idx1, idx2, idx_out) -> torch.Tensor:
             n *= int(t.shape[idx.index(lab)])
         return n
 
-    A3 = a.permute(*[idx1.index(lab) for lab in batch + rows + summed]).reshape(
+    A3 = a.permute(tuple(idx1.index(lab) for lab in batch + rows + summed)).reshape(
         size(a, idx1, batch), size(a, idx1, rows), size(a, idx1, summed))
-    B3 = b.permute(*[idx2.index(lab) for lab in batch + summed + cols]).reshape(
+    B3 = b.permute(tuple(idx2.index(lab) for lab in batch + summed + cols)).reshape(
         size(b, idx2, batch), size(b, idx2, summed), size(b, idx2, cols))
     C3 = zgemm_batched_hip(A3.contiguous(), B3.contiguous())
     dims = [int(a.shape[idx1.index(lab)]) for lab in batch + rows] + [int(b.shape[idx2.index(lab)]) for lab in cols]
@@ -210,7 +223,7 @@ def contract(array1, array2, idx1, idx2, idx_out) -> torch.Tensor:
     C = C3.reshape(dims)
     if not idx_out:
         return C
-    return C.permute(*[order.index(lab) for lab in idx_out]).contiguous()
+    return C.permute(tuple(order.index(lab) for lab in idx_out)).contiguous()
 
 
 def einsum_labels(string: str, ndims) -> tuple[list[list[str]], list[str]]:
@@ -268,12 +281,12 @@ def _take_diagonals(t, labs):
 def einsum(string: str, *tensors) -> torch.Tensor:
     """``math.einsum`` (backend_manager.py:526-548) on the GPU, with numpy's
     string rules (``...``, implicit output, a label repeated within an operand
-    for its diagonal, size-1 ellipsis axes broadcast).  Operands are contracted
+    for its diagonal, size-1 axes broadcast).  Operands are contracted
     pairwise, each pair as one batched product on the matrix cores
     (``contract``), in a greedy order: the pair whose result is smallest first,
     as opt_einsum's "greedy" prefers; an intermediate keeps the labels the
-    output or another operand still needs.  Other shared axes must have equal
-    sizes (ValueError otherwise, as numpy)."""
+    output or another operand still needs.  Axes of one label must have equal
+    sizes or size 1 (ValueError otherwise, as numpy)."""
     if not tensors:
         raise ValueError("einsum needs at least one operand")
     ts = [as_device_c128(t) for t in tensors]
@@ -283,19 +296,24 @@ def einsum(string: str, *tensors) -> torch.Tensor:
     every = set().union(*labels)
     if not set(outl).issubset(every):
         raise ValueError("Output labels must be present in input labels.")
+    for k, (t, labs) in enumerate(zip(ts, labels)):
+        for lab in set(labs):
+            dims = {int(n) for x, n in zip(labs, t.shape) if x == lab}
+            if len(dims) > 1:
+                raise ValueError(f"einsum {string!r}: the repeated axis {lab!r} of operand {k} has sizes {sorted(dims)}")
+    pairs = [_take_diagonals(t, labs) for t, labs in zip(ts, labels)]
     sizes = {}
-    for t, labs in zip(ts, labels):
+    for t, labs in pairs:
         for lab, n in zip(labs, t.shape):
             n = int(n)
             m = sizes.setdefault(lab, n)
             if m != n:
-                if not (lab.startswith("...") and 1 in (m, n)):
+                if 1 not in (m, n):
                     raise ValueError(f"einsum {string!r}: axis {lab!r} has sizes {m} and {n}")
                 sizes[lab] = max(m, n)
-    # size-1 ellipsis axes broadcast against the others, as numpy's do
-    ts = [t.expand(*[sizes[lab] for lab in labs]) if any(int(n) != sizes[lab] for lab, n in zip(labs, t.shape)) else t
-          for t, labs in zip(ts, labels)]
-    pairs = [_take_diagonals(t, labs) for t, labs in zip(ts, labels)]
+    # size-1 axes broadcast against the others, as numpy's do
+    pairs = [(t.expand(*[sizes[lab] for lab in labs]) if any(int(n) != sizes[lab] for lab, n in zip(labs, t.shape)) else t, labs)
+             for t, labs in pairs]
     # greedy order: contract the pair whose result is smallest (ties: leftmost)
     while len(pairs) > 1:
         best = None
@@ -322,4 +340,4 @@ def einsum(string: str, *tensors) -> torch.Tensor:
         rl = [lab for lab in rl if lab in outl]
     if not outl:
         return res.reshape(())
-    return res.permute(*[rl.index(lab) for lab in outl]).contiguous()
+    return res.permute(tuple(rl.index(lab) for lab in outl)).contiguous()

@@ -47,15 +47,23 @@ __device__ __forceinline__ double2 caxpy(double w, double2 z, double2 acc) {
     return make_double2(fma(w, z.x, acc.x), fma(w, z.y, acc.y));
 }
 
+// Numerators that Markstein's correction below cannot take: tiny non-zero
+// ones (the remainder would underflow) and infinite ones (x - s q is then
+// inf - inf).  They take the plain IEEE division, which keeps +-inf / s = +-inf.
+__device__ __forceinline__ bool div_rare(double x) {
+    const double ax = fabs(x);
+    return ((ax < 0x1p-960) && (x != 0.0)) || (ax > 0x1.fffffffffffffp+1023);
+}
+
 // Correctly rounded x / s given r = RN(1/s): Markstein's correction
 // q = RN(x r); q' = RN(q + RN(x - s q) r)  (P. Markstein, IBM J. Res. Dev.
 // 34(1), 1990; the theorem needs r correctly rounded and no underflow of the
-// remainder, so tiny numerators take the plain IEEE division instead).
+// remainder).  s = sqrt(k) >= 1, so a finite numerator never overflows.
 __device__ __forceinline__ double div_by(double x, double s, double r) {
     double q = x * r;
     double e = fma(-q, s, x);
     double qq = fma(e, r, q);
-    if (__builtin_expect((fabs(x) < 0x1p-960) & (x != 0.0), 0)) qq = x / s;
+    if (__builtin_expect(div_rare(x), 0)) qq = x / s;
     return qq;
 }
 
@@ -165,13 +173,13 @@ __device__ __forceinline__ void cdiv_many(double2 (&v)[N], double s, double r) {
         out[e].x = fma(ex, r, qx);
         out[e].y = fma(ey, r, qy);
         // exact zeros come out right (up to the sign of zero) and need no fix-up
-        tiny |= ((fabs(v[e].x) < 0x1p-960) & (v[e].x != 0.0)) | ((fabs(v[e].y) < 0x1p-960) & (v[e].y != 0.0));
+        tiny = tiny || div_rare(v[e].x) || div_rare(v[e].y);
     }
     if (__builtin_expect(tiny, 0)) {
 #pragma unroll
         for (int e = 0; e < N; ++e) {
-            if (fabs(v[e].x) < 0x1p-960) out[e].x = v[e].x / s;
-            if (fabs(v[e].y) < 0x1p-960) out[e].y = v[e].y / s;
+            if (div_rare(v[e].x)) out[e].x = v[e].x / s;
+            if (div_rare(v[e].y)) out[e].y = v[e].y / s;
         }
     }
 #pragma unroll

@@ -4,95 +4,19 @@
 // backend_manager.py:674-689 plus the shape/D checks the reference leaves to
 // chance), choice of execution path per triple, kernel launches.  Device
 // state is one sqrt / 1/sqrt table per device, created on first use.
-#include <hip/hip_runtime.h>
 
-#include <algorithm>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <map>
-#include <mutex>
-#include <string>
-#include <vector>
-
-#include "../../include/mm_hermite.h"
+#include "host.hpp"
 #include "common.hpp"
 #include "global_kernels.hpp"
 #include "jobs.hpp"
 #include "lds_kernel.hpp"
-#include "zgemm.hpp"
-#include "autoshape.hpp"
+#include "lds_launch.hpp"
 #include "binomial.hpp"
-#include "diagonal.hpp"
-#include "diagonal_vjp.hpp"
-#include "leftover.hpp"
-#include "leftover_vjp.hpp"
 #include "rows_dot.hpp"
 
 using namespace mm;
 
 namespace {
-
-thread_local std::string g_err = "no error";
-
-int fail(int code, const char *fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess) return fail(MM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-#define LAUNCH_CHECK(what)                                                                      \
-    do {                                                                                        \
-        hipError_t e_ = hipGetLastError();                                                      \
-        if (e_ != hipSuccess) return fail(MM_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e_)); \
-    } while (0)
-
-struct DevState {
-    double *sq = nullptr, *rsq = nullptr;
-    int max_lds = 65536;
-    int num_cu = 1;
-};
-
-std::mutex g_mu;
-std::map<int, DevState> g_dev;
-
-int device_state(DevState **out) {
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_dev.find(dev);
-    if (it != g_dev.end()) {
-        *out = &it->second;
-        return MM_OK;
-    }
-    DevState st;
-    std::vector<double> sq(MM_MAX_CUTOFF), rsq(MM_MAX_CUTOFF);
-    for (int n = 0; n < MM_MAX_CUTOFF; ++n) {
-        sq[n] = std::sqrt(double(n));  // correctly rounded, as np.sqrt (core.py:22)
-        rsq[n] = n ? 1.0 / sq[n] : 0.0;
-    }
-    HIP_TRY(hipMalloc(&st.sq, sizeof(double) * MM_MAX_CUTOFF));
-    HIP_TRY(hipMalloc(&st.rsq, sizeof(double) * MM_MAX_CUTOFF));
-    HIP_TRY(hipMemcpy(st.sq, sq.data(), sizeof(double) * MM_MAX_CUTOFF, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(st.rsq, rsq.data(), sizeof(double) * MM_MAX_CUTOFF, hipMemcpyHostToDevice));
-    HIP_TRY(hipDeviceGetAttribute(&st.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    HIP_TRY(hipDeviceGetAttribute(&st.num_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    st.max_lds -= 256;  // headroom for the kernels' static LDS
-    auto ins = g_dev.emplace(dev, st);
-    *out = &ins.first->second;
-    return MM_OK;
-}
 
 // ------------------------------------------------------------------ geometry
 
@@ -235,11 +159,6 @@ int choose_top(const Shape &S, bool fused, int max_lds, int maxt, int *threads) 
     return S.D - 1;
 }
 
-template <typename K>
-void raise_lds_limit(K kernel, int bytes) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-}
-
 // Forward-only uniform batches of D <= 3 triples whose level-0 slab has
 // LEAN_MIN_S .. LEAN_MAX_S points: one point per thread of a workgroup of up
 // to 1024 threads, in lds_block_kernel (about 70 registers: four waves per
@@ -259,39 +178,18 @@ int launch_lean(const LdsArgs &a0, const Shape &S, bool fused, int64_t B, const 
     LdsArgs a = a0;
     a.tmpl = make_job(S, 0, KIND_BLOCK);
     const int grid = int(std::min<int64_t>(B, int64_t(1) << 30));
-#define MM_LEAN(DD, FF)                                                                                  \
-    do {                                                                                                 \
-        raise_lds_limit(lds_block_kernel<DD, FF, 1024, 1>, int(shmem));                                  \
-        hipLaunchKernelGGL((lds_block_kernel<DD, FF, 1024, 1>), dim3(grid), dim3(T), size_t(shmem), s, a); \
-    } while (0)
-    if (S.D == 2)
-        MM_LEAN(2, false);
-    else
-        MM_LEAN(3, false);
-#undef MM_LEAN
+    launch_lds_block(S.D, a, dim3(grid), dim3(T), size_t(shmem), s);
     LAUNCH_CHECK("lds_block_kernel");
     return MM_OK;
 }
 
 // Launch the LDS kernel instantiation that covers D / fused.
 int launch_lds(const LdsArgs &a, int D, bool fused, int grid, int threads, int64_t shmem, hipStream_t s) {
-    if (lds_kernel_dmax8(D)) {
-        if (fused) {
-            raise_lds_limit(lds_kernel<8, true, 256>, int(shmem));
-            hipLaunchKernelGGL((lds_kernel<8, true, 256>), dim3(grid), dim3(threads), size_t(shmem), s, a);
-        } else {
-            raise_lds_limit(lds_kernel<8, false, 512>, int(shmem));
-            hipLaunchKernelGGL((lds_kernel<8, false, 512>), dim3(grid), dim3(threads), size_t(shmem), s, a);
-        }
-    } else {
-        if (fused) {
-            raise_lds_limit(lds_kernel<3, true, 512>, int(shmem));
-            hipLaunchKernelGGL((lds_kernel<3, true, 512>), dim3(grid), dim3(threads), size_t(shmem), s, a);
-        } else {
-            raise_lds_limit(lds_kernel<3, false, 768>, int(shmem));
-            hipLaunchKernelGGL((lds_kernel<3, false, 768>), dim3(grid), dim3(threads), size_t(shmem), s, a);
-        }
-    }
+    const dim3 g(grid), t(threads);
+    if (lds_kernel_dmax8(D))
+        (fused ? launch_lds_8f : launch_lds_8)(a, g, t, size_t(shmem), s);
+    else
+        (fused ? launch_lds_3f : launch_lds_3)(a, g, t, size_t(shmem), s);
     LAUNCH_CHECK("lds_kernel");
     return MM_OK;
 }
@@ -552,7 +450,6 @@ int recompute_blocks(const Shape &S, const DevState *ds) {
     return int(std::max<int64_t>(1, std::min<int64_t>((S0 + 1023) / 1024, int64_t(ds->num_cu) * 8)));
 }
 
-size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 size_t recompute_ws_bytes(const Shape &S, const DevState *ds) {
     int T = 64;
@@ -721,309 +618,6 @@ void launch_binomial_kernels(const Shape &S, const double2 *c, double max_l2, in
     hipLaunchKernelGGL(binomial_stop_kernel, dim3(1), dim3(256), size_t(W) * 8, s, part, nblk, W, c, max_l2, global_cutoff, wstop, norm_out);
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((S.N + 255) / 256, int64_t(num_cu) * 16));
     hipLaunchKernelGGL((weight_mask_kernel<D>), dim3(unsigned(grid)), dim3(256), 0, s, G, g, wstop);
-}
-
-// ------------------------------------------------------------ diagonal path
-
-struct DiagGeom {
-    DiagArgs a;
-    int M = 0;
-    int64_t n2 = 0, n11 = 0;  // entries of the d >= 1 arr2 and arr1010 (each), per batch column
-    size_t t_bytes = 0, bytes = 0;
-};
-
-int diag_geom(const int64_t *cutoffs, int M, int64_t nb, DiagGeom *out) {
-    if (!cutoffs) return fail(MM_ERR_INVALID, "cutoffs pointer is NULL");
-    if (M < 1 || M > DG_MAXM) return fail(MM_ERR_INVALID, "diagonal needs 1 <= M <= %d modes, got %d", DG_MAXM, M);
-    if (nb < 1) return fail(MM_ERR_INVALID, "batch of %lld columns", (long long)nb);
-    DiagGeom g;
-    memset(&g.a, 0, sizeof(g.a));
-    g.M = M;
-    int64_t P = 1, S = 0;
-    for (int j = 0; j < M; ++j) {
-        if (cutoffs[j] < 1 || cutoffs[j] + 2 >= MM_MAX_CUTOFF)
-            return fail(MM_ERR_INVALID, "cutoffs[%d]=%lld outside [1, %d)", j, (long long)cutoffs[j], MM_MAX_CUTOFF - 2);
-        g.a.cut[j] = int(cutoffs[j]);
-        if (P > (int64_t(1) << 40) / (cutoffs[j] * nb)) return fail(MM_ERR_INVALID, "lattice too large");
-        P *= cutoffs[j];
-        S += cutoffs[j] - 1;
-    }
-    if (P > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "diagonal: prod(cutoffs) = %lld is not below 2^31", (long long)P);
-    g.a.P = P;
-    g.a.nb = nb;
-    g.a.S = int(S);
-    int64_t len = 1;
-    for (int j = M - 1; j >= 0; --j) {
-        g.a.stride[j] = len;
-        len *= g.a.cut[j];
-        g.a.len[j] = len;
-    }
-    for (int d = 1; d < M; ++d) {  // sub-lattices of the off-diagonal pivots d >= 1
-        g.a.off2[d] = g.n2;
-        g.a.off11[d] = g.n11;
-        g.n2 += g.a.len[d];
-        g.n11 += int64_t(M - d - 1) * g.a.len[d];
-    }
-    g.t_bytes = align256(size_t(2 * (M + 1)) * size_t(S + 1) * 8) + align256(size_t(P) * 4) +
-                align256(size_t(M) * size_t(P) * 4);  // T and Q, list, nbr
-    g.bytes = g.t_bytes + size_t(16) * size_t(nb) * size_t(4 * M * P + g.n2 + 2 * g.n11);  // two inboxes + sub-lattices
-    *out = g;
-    return MM_OK;
-}
-
-// T[0][w] for every weight, as the device table holds it (prefix sums per mode)
-std::vector<int64_t> diag_weight_counts(const DiagArgs &a, int M) {
-    const int W = a.S + 1;
-    std::vector<int64_t> row(W, 0), pre(W);
-    row[0] = 1;
-    for (int j = M - 1; j >= 0; --j) {
-        int64_t acc = 0;
-        for (int s = 0; s < W; ++s) pre[s] = (acc += row[s]);
-        for (int s = 0; s < W; ++s) row[s] = pre[s] - (s >= a.cut[j] ? pre[s - a.cut[j]] : 0);
-    }
-    return row;
-}
-
-// Bytes of LDS for the count and prefix tables in each rank block, or 0 when
-// they are too large and come from diag_counts_kernel in global memory.
-size_t diag_table_lds(const DiagArgs &a, int M) {
-    const size_t b = size_t(2 * (M + 1)) * size_t(a.S + 1) * 8;
-    return b <= 48 * 1024 ? b : 0;
-}
-
-template <int M, bool REORDER>
-int launch_diag_weights(const DiagArgs &a, const std::vector<int64_t> &counts, int num_cu, hipStream_t s) {
-    constexpr int PB = DG_THREADS / (2 * M);
-    const size_t lds = diag_table_lds(a, M);
-    const int64_t rgrid = std::max<int64_t>(1, std::min<int64_t>((a.P + 255) / 256, int64_t(num_cu) * (lds ? 4 : 16)));
-    hipLaunchKernelGGL((diag_rank_kernel<M>), dim3(unsigned(rgrid)), dim3(256), lds, s, a, lds ? 1 : 0);
-    int64_t start = 0;
-    for (int w = 0; w <= a.S; ++w) {
-        const int64_t n = counts[w] * a.nb;
-        const int64_t blocks = (n + PB - 1) / PB;
-        if (blocks > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "weight %d has too many params for one launch", w);
-        if (blocks > 0)
-            hipLaunchKernelGGL((diag_group_kernel<M, REORDER>), dim3(unsigned(blocks)), dim3(DG_THREADS), 0, s, a, w, counts[w], start);
-        start += counts[w];
-    }
-    return MM_OK;
-}
-
-// Device pointers of the forward's workspace: T and Q, list, fwd, inboxes, d >= 1 sub-lattices.
-void diag_bind(DiagGeom &g, const double *A, const double *B, const double *c, int c_batched, double *G, char *ws,
-               const DevState *ds) {
-    DiagArgs &a = g.a;
-    const int M = g.M;
-    const int64_t nb = a.nb;
-    a.A = reinterpret_cast<const double2 *>(A);
-    a.B = reinterpret_cast<const double2 *>(B);
-    a.c = reinterpret_cast<const double2 *>(c);
-    a.c_batched = c_batched ? 1 : 0;
-    a.sq = ds->sq;
-    a.rsq = ds->rsq;
-    a.T = reinterpret_cast<int64_t *>(ws);
-    a.Q = a.T + int64_t(M + 1) * (a.S + 1);
-    a.list = reinterpret_cast<int32_t *>(ws + align256(size_t(2 * (M + 1)) * size_t(a.S + 1) * 8));
-    a.fwd = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(a.list) + align256(size_t(a.P) * 4));
-    a.arr0 = reinterpret_cast<double2 *>(G);
-    a.inD = reinterpret_cast<double2 *>(ws + g.t_bytes);
-    a.inO = a.inD + 2 * M * a.P * nb;
-    a.arr2 = a.inO + 2 * M * a.P * nb;
-    a.a1010 = a.arr2 + g.n2 * nb;
-    a.a1001 = a.a1010 + g.n11 * nb;
-}
-
-int diag_forward(DiagGeom &g, int reordered, const DevState *ds, hipStream_t s, std::vector<int64_t> *counts_out) {
-    DiagArgs &a = g.a;
-    const int M = g.M;
-    if (diag_table_lds(a, M) == 0) {
-        hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(64), 0, s, a, M);
-        LAUNCH_CHECK("diag_counts_kernel");
-    }
-    const std::vector<int64_t> counts = diag_weight_counts(a, M);
-    int rc = MM_OK;
-    switch (M * 2 + (reordered ? 1 : 0)) {
-#define CASE(MM)                                                       \
-    case 2 * MM: rc = launch_diag_weights<MM, false>(a, counts, ds->num_cu, s); break; \
-    case 2 * MM + 1: rc = launch_diag_weights<MM, true>(a, counts, ds->num_cu, s); break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-#undef CASE
-        default:
-            return fail(MM_ERR_INVALID, "bad M");
-    }
-    if (rc) return rc;
-    LAUNCH_CHECK("diag_group_kernel");
-    if (counts_out) *counts_out = counts;
-    return MM_OK;
-}
-
-// The VJP's blocks: one per PB params of the largest weight, at most 1024.
-int diag_vjp_blocks(const DiagGeom &g) {
-    const std::vector<int64_t> counts = diag_weight_counts(g.a, g.M);
-    const int64_t mx = *std::max_element(counts.begin(), counts.end());
-    const int PB = DG_THREADS / (2 * g.M);
-    return int(std::max<int64_t>(1, std::min<int64_t>(1024, (mx + PB - 1) / PB)));
-}
-
-size_t diag_vjp_bytes(const DiagGeom &g, int nblk) {
-    const int G = 2 * g.M;
-    const size_t mirror = size_t(16) * size_t(2 * G * g.a.P + g.n2 + 2 * g.n11);
-    return align256(g.bytes) + align256(mirror) + size_t(16) * size_t(nblk) * size_t(G) * size_t(G + 1);
-}
-
-template <int M, bool REORDER>
-void launch_diag_vjp(DiagVjpArgs &v, const std::vector<int64_t> &counts, int nblk, double2 *dA, double2 *dB, hipStream_t s) {
-    constexpr int PB = DG_THREADS / (2 * M);
-    const int64_t np = int64_t(nblk) * 2 * M * (2 * M + 1);
-    hipLaunchKernelGGL(zero_c128_span_kernel, dim3(64), dim3(256), 0, s, v.part, np);
-    std::vector<int64_t> starts(counts.size(), 0);
-    for (size_t w = 1; w < counts.size(); ++w) starts[w] = starts[w - 1] + counts[w - 1];
-    for (int w = v.a.S; w >= 0; --w) {
-        // blocks 0 .. nw-1 of the fixed nblk slots: block b always adds to partial row b
-        const int nw = int(std::min<int64_t>(nblk, (counts[w] + PB - 1) / PB));
-        const int64_t per = int64_t(nw) * PB;
-        const int iters = int((counts[w] + per - 1) / per);
-        hipLaunchKernelGGL((diag_vjp_kernel<M, REORDER>), dim3(nw), dim3(DG_THREADS), 0, s, v, w, counts[w], starts[w], iters);
-    }
-    hipLaunchKernelGGL((diag_vjp_finalize<M, REORDER>), dim3(2 * M * (2 * M + 1)), dim3(256), 0, s, v.part, nblk, dA, dB);
-}
-
-// ------------------------------------------------------------ one leftover mode
-
-constexpr int LO_MAXT = 7;  // PNR-detected modes
-
-int lo_geom(int64_t output_cutoff, const int64_t *pnr, int Mt, DiagGeom *g, int *c0, size_t *bytes) {
-    if (!pnr) return fail(MM_ERR_INVALID, "pnr_cutoffs pointer is NULL");
-    if (Mt < 1) return fail(MM_ERR_INVALID, "The number of modes should be greater than 1.");
-    if (Mt > LO_MAXT) return fail(MM_ERR_UNSUPPORTED, "1leftoverMode supports up to %d detected modes, got %d", LO_MAXT, Mt);
-    if (output_cutoff < 0) return fail(MM_ERR_INVALID, "output_cutoff=%lld is negative", (long long)output_cutoff);
-    if (output_cutoff >= 1024) return fail(MM_ERR_UNSUPPORTED, "output_cutoff=%lld: at most 1023", (long long)output_cutoff);
-    int64_t tail[DG_MAXM];
-    for (int j = 0; j < Mt; ++j) {
-        if (pnr[j] < 0) return fail(MM_ERR_INVALID, "pnr_cutoffs[%d]=%lld is negative", j, (long long)pnr[j]);
-        tail[j] = pnr[j] + 1;
-    }
-    *c0 = int(output_cutoff + 1);
-    int rc = diag_geom(tail, Mt, int64_t(*c0) * *c0, g);
-    if (rc) return rc;
-    *bytes = g->bytes + size_t(16) * size_t(Mt) * size_t(g->a.P) * size_t(*c0) * size_t(*c0);  // + piv
-    return MM_OK;
-}
-
-template <int MT, bool REORDER>
-int launch_leftover(LeftArgs &a, const std::vector<int64_t> &counts, int num_cu, hipStream_t s) {
-    constexpr int PB = DG_THREADS / (2 * MT);
-    const size_t lds = diag_table_lds(a.t, MT);
-    const int64_t rgrid = std::max<int64_t>(1, std::min<int64_t>((a.t.P + 255) / 256, int64_t(num_cu) * (lds ? 4 : 16)));
-    hipLaunchKernelGGL((diag_rank_kernel<MT>), dim3(unsigned(rgrid)), dim3(256), lds, s, a.t, lds ? 1 : 0);
-    const int ithreads = (a.c0 + 63) / 64 * 64;
-    hipLaunchKernelGGL((lo_init_kernel<MT, REORDER>), dim3(1), dim3(ithreads), size_t(3) * a.c0 * 16, s, a);
-    int64_t start = 0;
-    for (int w = 0; w <= a.t.S; ++w) {
-        const int64_t blocks = (counts[w] * a.C + PB - 1) / PB;
-        if (blocks > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "weight %d has too many entries for one launch", w);
-        if (blocks > 0) {
-            hipLaunchKernelGGL((lo_diag_kernel<MT, REORDER>), dim3(unsigned(blocks)), dim3(DG_THREADS), 0, s, a, w, counts[w], start);
-            hipLaunchKernelGGL((lo_off_kernel<MT, REORDER>), dim3(unsigned(blocks)), dim3(DG_THREADS), 0, s, a, w, counts[w], start);
-        }
-        start += counts[w];
-    }
-    return MM_OK;
-}
-
-// Binds the forward's arrays to the workspace (layout of lo_geom).
-void lo_bind(const DiagGeom &g, int c0, const double *A, const double *b, const double *c, double *out, char *ws, DevState *ds,
-             LeftArgs &a) {
-    const int Mt = g.M;
-    a.t = g.a;
-    DiagArgs &t = a.t;
-    t.sq = ds->sq;
-    t.rsq = ds->rsq;
-    const int64_t C = int64_t(c0) * c0;
-    t.T = reinterpret_cast<int64_t *>(ws);
-    t.Q = t.T + int64_t(Mt + 1) * (t.S + 1);
-    t.list = reinterpret_cast<int32_t *>(ws + align256(size_t(2 * (Mt + 1)) * size_t(t.S + 1) * 8));
-    t.fwd = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(t.list) + align256(size_t(t.P) * 4));
-    t.inD = reinterpret_cast<double2 *>(ws + g.t_bytes);
-    t.inO = t.inD + 2 * Mt * t.P * C;
-    t.arr2 = t.inO + 2 * Mt * t.P * C;
-    t.a1010 = t.arr2 + g.n2 * C;
-    t.a1001 = t.a1010 + g.n11 * C;
-    t.arr0 = nullptr;
-    a.piv = reinterpret_cast<double2 *>(ws + g.bytes);
-    a.A = reinterpret_cast<const double2 *>(A);
-    a.B = reinterpret_cast<const double2 *>(b);
-    a.c = reinterpret_cast<const double2 *>(c);
-    a.c0 = c0;
-    a.C = int(C);
-    a.out = reinterpret_cast<double2 *>(out);
-}
-
-int lo_forward(LeftArgs &a, int Mt, int reordered, DevState *ds, hipStream_t s, std::vector<int64_t> *counts_out) {
-    if (diag_table_lds(a.t, Mt) == 0) {
-        hipLaunchKernelGGL(diag_counts_kernel, dim3(1), dim3(64), 0, s, a.t, Mt);
-        LAUNCH_CHECK("diag_counts_kernel");
-    }
-    const std::vector<int64_t> counts = diag_weight_counts(a.t, Mt);
-    int rc = MM_OK;
-    switch (Mt * 2 + (reordered ? 1 : 0)) {
-#define CASE(MM)                                                                   \
-    case 2 * MM: rc = launch_leftover<MM, false>(a, counts, ds->num_cu, s); break; \
-    case 2 * MM + 1: rc = launch_leftover<MM, true>(a, counts, ds->num_cu, s); break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
-#undef CASE
-        default:
-            return fail(MM_ERR_INVALID, "bad number of detected modes");
-    }
-    if (rc) return rc;
-    LAUNCH_CHECK("lo kernels");
-    if (counts_out) *counts_out = counts;
-    return MM_OK;
-}
-
-// The VJP's blocks: one per PB groups (params, block entry) of the largest weight, at most 1024.
-int lo_vjp_blocks(const DiagGeom &g, int Mt, int64_t C) {
-    const std::vector<int64_t> counts = diag_weight_counts(g.a, Mt);
-    const int64_t mx = *std::max_element(counts.begin(), counts.end());
-    const int PB = DG_THREADS / (2 * Mt);
-    return int(std::max<int64_t>(1, std::min<int64_t>(1024, (mx * C + PB - 1) / PB)));
-}
-
-// forward workspace, adjoint mirrors of the inboxes and sub-lattices, adP, partial rows
-size_t lo_vjp_bytes(const DiagGeom &g, int Mt, int64_t C, size_t fwd_bytes, int nblk) {
-    const int GL = 2 * (Mt + 1);
-    const size_t mirror = size_t(16) * size_t(C) * size_t(4 * Mt * g.a.P + g.n2 + 2 * g.n11);
-    const size_t adp = size_t(16) * size_t(Mt) * size_t(g.a.P) * size_t(C);
-    return align256(fwd_bytes) + align256(mirror) + align256(adp) + size_t(16) * size_t(nblk + 1) * GL * (GL + 1);
-}
-
-template <int MT, bool REORDER>
-void launch_lo_vjp(LeftVjpArgs &v, const std::vector<int64_t> &counts, int nblk, double2 *dA, double2 *dB, hipStream_t s) {
-    constexpr int PB = DG_THREADS / (2 * MT);
-    constexpr int GL = 2 * (MT + 1);
-    const int64_t C = v.a.C, P = v.a.t.P;
-    auto zero = [&](double2 *p, int64_t n) {
-        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256));
-        hipLaunchKernelGGL(zero_c128_span_kernel, dim3(unsigned(grid)), dim3(256), 0, s, p, n);
-    };
-    // the mirrors, adP and the partial rows are one contiguous span of the workspace
-    zero(v.adD, (v.part + int64_t(nblk + 1) * GL * (GL + 1)) - v.adD);
-    std::vector<int64_t> starts(counts.size(), 0);
-    for (size_t w = 1; w < counts.size(); ++w) starts[w] = starts[w - 1] + counts[w - 1];
-    for (int w = v.a.t.S; w >= 0; --w) {
-        const int64_t groups = counts[w] * C;
-        if (groups == 0) continue;
-        // blocks 0 .. nw-1 of the fixed nblk slots: block b always adds to partial row b
-        const int nw = int(std::min<int64_t>(nblk, (groups + PB - 1) / PB));
-        const int64_t per = int64_t(nw) * PB;
-        const int iters = int((groups + per - 1) / per);
-        hipLaunchKernelGGL((lo_off_vjp_kernel<MT, REORDER>), dim3(nw), dim3(DG_THREADS), 0, s, v, counts[w], starts[w], iters);
-        hipLaunchKernelGGL((lo_diag_vjp_kernel<MT, REORDER>), dim3(nw), dim3(DG_THREADS), 0, s, v, counts[w], starts[w], iters);
-    }
-    int nt = 64;
-    while (nt < v.a.c0) nt <<= 1;  // c0 <= 1024
-    hipLaunchKernelGGL((lo_init_vjp_kernel<MT, REORDER>), dim3(1), dim3(nt), 0, s, v, nblk);
-    hipLaunchKernelGGL((diag_vjp_finalize<MT + 1, REORDER>), dim3(GL * (GL + 1)), dim3(256), 0, s, v.part, nblk + 1, dA, dB);
 }
 
 // ------------------------------------------- conversion fused with the contraction
@@ -1502,82 +1096,6 @@ int mm_hermite_binomial(const double *A, const double *b, const double *c, const
     return MM_OK;
 }
 
-int mm_zgemm_batched(const double *A, int64_t lda, int64_t sA, const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc,
-                     int64_t sC, int64_t M, int64_t N, int64_t K, int64_t batch, void *stream) {
-    if (M < 0 || N < 0 || K < 0 || batch < 0) return fail(MM_ERR_INVALID, "negative matrix size or batch");
-    if (M == 0 || N == 0 || batch == 0) return MM_OK;
-    if (!A || !B || !C) return fail(MM_ERR_INVALID, "NULL matrix pointer");
-    if (lda < K || ldb < N || ldc < N) return fail(MM_ERR_INVALID, "leading dimension smaller than the row length");
-    if (sA < 0 || sB < 0 || sC < 0) return fail(MM_ERR_INVALID, "negative batch stride");
-    if (batch > 1 && sC < (M - 1) * ldc + N) return fail(MM_ERR_INVALID, "the products of a batch overlap in C");
-    const bool tall = M > 64 && M <= 128;  // one 128-row tile covers M: B is read once
-    const int bm = tall ? 128 : 64;
-    const int64_t gx = (N + ZG_BN - 1) / ZG_BN, gy = (M + bm - 1) / bm;
-    if (gx > 0x7fffffff || gy > 65535) return fail(MM_ERR_UNSUPPORTED, "matrix too large for one launch");
-    ZgemmArgs a;
-    a.M = M;
-    a.N = N;
-    a.K = K;
-    a.lda = lda;
-    a.ldb = ldb;
-    a.ldc = ldc;
-    a.sA = sA;
-    a.sB = sB;
-    a.sC = sC;
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    for (int64_t b0 = 0; b0 < batch; b0 += 65535) {  // grid z holds at most 65535 products
-        const int64_t nz = std::min<int64_t>(65535, batch - b0);
-        a.A = reinterpret_cast<const double2 *>(A) + b0 * sA;
-        a.B = reinterpret_cast<const double2 *>(B) + b0 * sB;
-        a.C = reinterpret_cast<double2 *>(C) + b0 * sC;
-        if (tall)
-            hipLaunchKernelGGL(zgemm_mfma_kernel<128>, dim3(unsigned(gx), unsigned(gy), unsigned(nz)), dim3(512), 0, s, a);
-        else
-            hipLaunchKernelGGL(zgemm_mfma_kernel<64>, dim3(unsigned(gx), unsigned(gy), unsigned(nz)), dim3(256), 0, s, a);
-    }
-    LAUNCH_CHECK("zgemm_mfma_kernel");
-    return MM_OK;
-}
-
-int mm_zgemm(const double *A, int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, int64_t M, int64_t N, int64_t K,
-             void *stream) {
-    return mm_zgemm_batched(A, lda, 0, B, ldb, 0, C, ldc, 0, M, N, K, 1, stream);
-}
-
-size_t mm_autoshape_workspace(int64_t B, int M) { return B > 0 && M > 0 ? size_t(B) * size_t(M) * 8 * 16 : 0; }
-
-int mm_autoshape(const double *A, const double *b, const double *c, int64_t B, int M, double max_prob, int max_shape,
-                 int min_shape, int64_t *shape, void *workspace, size_t ws_bytes, void *stream) {
-    if (M < 1 || M > AS_MAXM) return fail(MM_ERR_INVALID, "autoshape needs 1 <= M <= %d modes, got %d", AS_MAXM, M);
-    if (B < 0) return fail(MM_ERR_INVALID, "negative batch size");
-    if (max_shape < 1 || int64_t(max_shape) + 2 >= MM_MAX_CUTOFF)
-        return fail(MM_ERR_INVALID, "max_shape must be in [1, %d)", MM_MAX_CUTOFF - 2);
-    if (B == 0) return MM_OK;
-    if (!A || !b || !c || !shape) return fail(MM_ERR_INVALID, "NULL array pointer");
-    if (B * M > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "too many states for one launch");
-    if (!workspace || ws_bytes < mm_autoshape_workspace(B, M))
-        return fail(MM_ERR_WORKSPACE, "autoshape workspace of %zu bytes needed, %zu given", mm_autoshape_workspace(B, M), ws_bytes);
-    DevState *ds = nullptr;
-    int rc = device_state(&ds);
-    if (rc) return rc;
-    AutoshapeArgs a;
-    a.marg = reinterpret_cast<double2 *>(workspace);
-    a.A = reinterpret_cast<const double2 *>(A);
-    a.b = reinterpret_cast<const double2 *>(b);
-    a.c = reinterpret_cast<const double2 *>(c);
-    a.B = B;
-    a.M = M;
-    a.max_prob = max_prob;
-    a.max_shape = max_shape;
-    a.min_shape = min_shape;
-    a.sq = ds->sq;
-    a.shape = shape;
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(autoshape_schur_kernel, dim3(unsigned(B * M)), dim3(64), 0, s, a);
-    hipLaunchKernelGGL(autoshape_recursion_kernel, dim3(unsigned((B * M + 255) / 256)), dim3(256), 0, s, a);
-    LAUNCH_CHECK("autoshape kernels");
-    return MM_OK;
-}
 
 size_t mm_hermite_fwd_contract_workspace(const int64_t *shape, int D) {
     Shape S;
@@ -1627,157 +1145,6 @@ int mm_hermite_fwd_contract(const double *A, const double *b, const double *c, c
         rc = launch_rows_dot(slot[m % 3], w2, o2 + int64_t(m) * R, R, K, s);
         if (rc) return rc;
     }
-    return MM_OK;
-}
-
-size_t mm_hermite_1leftover_workspace(int64_t output_cutoff, const int64_t *pnr_cutoffs, int Mt) {
-    DiagGeom g;
-    int c0 = 0;
-    size_t bytes = 0;
-    return lo_geom(output_cutoff, pnr_cutoffs, Mt, &g, &c0, &bytes) ? 0 : bytes;
-}
-
-int mm_hermite_1leftover(const double *A, const double *b, const double *c, int64_t output_cutoff, const int64_t *pnr_cutoffs,
-                         int Mt, int reordered, double *out, void *workspace, size_t ws_bytes, void *stream) {
-    DiagGeom g;
-    int c0 = 0;
-    size_t need = 0;
-    int rc = lo_geom(output_cutoff, pnr_cutoffs, Mt, &g, &c0, &need);
-    if (rc) return rc;
-    if (!A || !b || !c || !out) return fail(MM_ERR_INVALID, "NULL array pointer");
-    if (!workspace || ws_bytes < need) return fail(MM_ERR_WORKSPACE, "1leftover workspace of %zu bytes needed, %zu given", need, ws_bytes);
-    DevState *ds = nullptr;
-    rc = device_state(&ds);
-    if (rc) return rc;
-    LeftArgs a;
-    lo_bind(g, c0, A, b, c, out, reinterpret_cast<char *>(workspace), ds, a);
-    std::vector<int64_t> counts;
-    return lo_forward(a, Mt, reordered, ds, reinterpret_cast<hipStream_t>(stream), &counts);
-}
-
-size_t mm_hermite_1leftover_vjp_workspace(int64_t output_cutoff, const int64_t *pnr_cutoffs, int Mt) {
-    DiagGeom g;
-    int c0 = 0;
-    size_t bytes = 0;
-    if (lo_geom(output_cutoff, pnr_cutoffs, Mt, &g, &c0, &bytes)) return 0;
-    return lo_vjp_bytes(g, Mt, int64_t(c0) * c0, bytes, lo_vjp_blocks(g, Mt, int64_t(c0) * c0));
-}
-
-int mm_hermite_1leftover_vjp(const double *A, const double *b, const double *c, int64_t output_cutoff, const int64_t *pnr_cutoffs,
-                             int Mt, int reordered, const double *dLdout, double *out, double *dA, double *db, double *dc,
-                             void *workspace, size_t ws_bytes, void *stream) {
-    DiagGeom g;
-    int c0 = 0;
-    size_t fwd_bytes = 0;
-    int rc = lo_geom(output_cutoff, pnr_cutoffs, Mt, &g, &c0, &fwd_bytes);
-    if (rc) return rc;
-    if (!A || !b || !c || !dLdout || !out || !dA || !db || !dc) return fail(MM_ERR_INVALID, "NULL array pointer");
-    const int64_t C = int64_t(c0) * c0;
-    const int nblk = lo_vjp_blocks(g, Mt, C);
-    const size_t need = lo_vjp_bytes(g, Mt, C, fwd_bytes, nblk);
-    if (!workspace || ws_bytes < need)
-        return fail(MM_ERR_WORKSPACE, "1leftover VJP workspace of %zu bytes needed, %zu given", need, ws_bytes);
-    DevState *ds = nullptr;
-    rc = device_state(&ds);
-    if (rc) return rc;
-    char *ws = reinterpret_cast<char *>(workspace);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    LeftVjpArgs v;
-    lo_bind(g, c0, A, b, c, out, ws, ds, v.a);
-    std::vector<int64_t> counts;
-    rc = lo_forward(v.a, Mt, reordered, ds, s, &counts);
-    if (rc) return rc;
-    const int64_t P = g.a.P;
-    const int64_t nmirror = C * (4 * Mt * P + g.n2 + 2 * g.n11);
-    v.cot = reinterpret_cast<const double2 *>(dLdout);
-    v.adD = reinterpret_cast<double2 *>(ws + align256(fwd_bytes));
-    v.adO = v.adD + 2 * Mt * P * C;
-    v.ad2 = v.adO + 2 * Mt * P * C;
-    v.ad1010 = v.ad2 + g.n2 * C;
-    v.ad1001 = v.ad1010 + g.n11 * C;
-    v.adP = reinterpret_cast<double2 *>(ws + align256(fwd_bytes) + align256(size_t(16) * size_t(nmirror)));
-    v.part = reinterpret_cast<double2 *>(reinterpret_cast<char *>(v.adP) + align256(size_t(16) * size_t(Mt * P * C)));
-    v.dc = reinterpret_cast<double2 *>(dc);
-    double2 *dA2 = reinterpret_cast<double2 *>(dA), *dB2 = reinterpret_cast<double2 *>(db);
-    switch (Mt * 2 + (reordered ? 1 : 0)) {
-#define CASE(MM)                                                                        \
-    case 2 * MM: launch_lo_vjp<MM, false>(v, counts, nblk, dA2, dB2, s); break; \
-    case 2 * MM + 1: launch_lo_vjp<MM, true>(v, counts, nblk, dA2, dB2, s); break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
-#undef CASE
-        default:
-            return fail(MM_ERR_INVALID, "bad number of detected modes");
-    }
-    LAUNCH_CHECK("lo vjp kernels");
-    return MM_OK;
-}
-
-size_t mm_hermite_diagonal_workspace(const int64_t *cutoffs, int M, int64_t nb) {
-    DiagGeom g;
-    return diag_geom(cutoffs, M, nb, &g) ? 0 : g.bytes;
-}
-
-int mm_hermite_diagonal(const double *A, const double *B, const double *c, int c_batched, const int64_t *cutoffs, int M,
-                        int64_t nb, int reordered, double *G, void *workspace, size_t ws_bytes, void *stream) {
-    DiagGeom g;
-    int rc = diag_geom(cutoffs, M, nb, &g);
-    if (rc) return rc;
-    if (!A || !B || !c || !G) return fail(MM_ERR_INVALID, "NULL array pointer");
-    if (!workspace || ws_bytes < g.bytes)
-        return fail(MM_ERR_WORKSPACE, "diagonal workspace of %zu bytes needed, %zu given", g.bytes, ws_bytes);
-    DevState *ds = nullptr;
-    rc = device_state(&ds);
-    if (rc) return rc;
-    diag_bind(g, A, B, c, c_batched, G, reinterpret_cast<char *>(workspace), ds);
-    return diag_forward(g, reordered, ds, reinterpret_cast<hipStream_t>(stream), nullptr);
-}
-
-size_t mm_hermite_diagonal_vjp_workspace(const int64_t *cutoffs, int M) {
-    DiagGeom g;
-    if (diag_geom(cutoffs, M, 1, &g)) return 0;
-    return diag_vjp_bytes(g, diag_vjp_blocks(g));
-}
-
-int mm_hermite_diagonal_vjp(const double *A, const double *B, const double *c, const int64_t *cutoffs, int M, int reordered,
-                            const double *dLdG, double *G, double *dA, double *dB, double *dc, void *workspace, size_t ws_bytes,
-                            void *stream) {
-    DiagGeom g;
-    int rc = diag_geom(cutoffs, M, 1, &g);
-    if (rc) return rc;
-    if (!A || !B || !c || !dLdG || !G || !dA || !dB || !dc) return fail(MM_ERR_INVALID, "NULL array pointer");
-    const int nblk = diag_vjp_blocks(g);
-    const size_t need = diag_vjp_bytes(g, nblk);
-    if (!workspace || ws_bytes < need) return fail(MM_ERR_WORKSPACE, "diagonal VJP workspace of %zu bytes needed, %zu given", need, ws_bytes);
-    DevState *ds = nullptr;
-    rc = device_state(&ds);
-    if (rc) return rc;
-    char *ws = reinterpret_cast<char *>(workspace);
-    diag_bind(g, A, B, c, 0, G, ws, ds);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    std::vector<int64_t> counts;
-    rc = diag_forward(g, reordered, ds, s, &counts);
-    if (rc) return rc;
-    DiagVjpArgs v;
-    v.a = g.a;
-    v.cot = reinterpret_cast<const double2 *>(dLdG);
-    v.adD = reinterpret_cast<double2 *>(ws + align256(g.bytes));
-    v.adO = v.adD + 2 * M * g.a.P;
-    v.ad2 = v.adO + 2 * M * g.a.P;
-    v.ad1010 = v.ad2 + g.n2;
-    v.ad1001 = v.ad1010 + g.n11;
-    v.part = reinterpret_cast<double2 *>(ws + align256(g.bytes) + align256(size_t(16) * size_t(4 * M * g.a.P + g.n2 + 2 * g.n11)));
-    v.dc = reinterpret_cast<double2 *>(dc);
-    double2 *dA2 = reinterpret_cast<double2 *>(dA), *dB2 = reinterpret_cast<double2 *>(dB);
-    switch (M * 2 + (reordered ? 1 : 0)) {
-#define CASE(MM)                                                                  \
-    case 2 * MM: launch_diag_vjp<MM, false>(v, counts, nblk, dA2, dB2, s); break; \
-    case 2 * MM + 1: launch_diag_vjp<MM, true>(v, counts, nblk, dA2, dB2, s); break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-#undef CASE
-        default:
-            return fail(MM_ERR_INVALID, "bad M");
-    }
-    LAUNCH_CHECK("diag_vjp_kernel");
     return MM_OK;
 }
 

@@ -104,6 +104,8 @@ def fock_array_derived(A, b, c, shape, fused: bool | None = None) -> torch.Tenso
     tracked = torch.is_grad_enabled() and any(t.requires_grad for t in (A, b, c))
     if fused and n >= 1 and not tracked and not settings.STABLE_FOCK_CONVERSION:
         nb = pymath.prod(batch_shape) if batch_shape else 1
+        if nb == 0:
+            return torch.empty(tuple(batch_shape) + shape, dtype=C128, device=A.device)
         A3 = A.reshape(nb, *A.shape[-2:])
         b3 = b.reshape(nb, b.shape[-1])
         c3 = c.reshape(nb, *shape_derived)

@@ -177,8 +177,9 @@ class BackendManager:
         r"""Einstein summation of Fock-basis arrays on the FP64 matrix cores
         (backend_manager.py:526-548), as ``ArrayAnsatz.contract`` and the
         derived-variables branch of ``fock_array`` call it: numpy's string
-        rules, operands contracted pairwise left to right.  ``optimize`` and
-        ``backend`` are accepted for the reference's signature."""
+        rules, operands contracted pairwise in a greedy order (the pair with
+        the smallest result first).  ``optimize`` and ``backend`` are accepted
+        for the reference's signature and ignored."""
         from ..contraction import einsum as _einsum
 
         return _einsum(string, *tensors)

@@ -200,3 +200,21 @@ def pnr_dm(m: int = 6, seed: int = 6):
     A, b, c = gaussian_ket(m, rng, r_max=0.6, b_sigma=0.5)
     dm = apply_loss(*ket_to_dm(A, b, c), rng.uniform(0.6, 1.0, m))
     return normalize_dm(*dm)
+
+
+def displacement_gate_triple(alpha: complex):
+    """(A, b, c) of the one-mode displacement gate D(alpha) on wires (out, in)
+    (triples.py:424-446): A = [[0, 1], [1, 0]], b = [alpha, -conj(alpha)],
+    c = exp(-|alpha|^2 / 2)."""
+    alpha = complex(alpha)
+    A = np.array([[0.0, 1.0], [1.0, 0.0]], np.complex128)
+    b = np.array([alpha, -np.conj(alpha)], np.complex128)
+    return A, b, np.complex128(np.exp(-abs(alpha) ** 2 / 2))
+
+
+def squeezing_gate_triple(r: float, phi: float = 0.0):
+    """(A, b, c) of the one-mode squeezing gate S(r, phi) on wires (out, in)
+    (triples.py:449-483)."""
+    t, s = np.tanh(r), 1.0 / np.cosh(r)
+    A = np.array([[-np.exp(1j * phi) * t, s], [s, np.exp(-1j * phi) * t]], np.complex128)
+    return A, np.zeros(2, np.complex128), np.complex128(1.0 / np.sqrt(np.cosh(r)))

@@ -46,7 +46,7 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [_i64p, ctypes.c_int, _dp, _dp, _dp, _dp]
             f.restype = ctypes.c_int
-        for name in ("oracle_vjp", "oracle_vjp_kahan"):
+        for name in ("oracle_vjp", "oracle_vjp_kahan", "oracle_vjp_kahan_par"):
             f = getattr(L, name)
             f.argtypes = [_i64p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
             f.restype = ctypes.c_int
@@ -102,11 +102,13 @@ def stable(shape, A, b, c, out: np.ndarray | None = None) -> np.ndarray:
     return G
 
 
-def vanilla_vjp(G, c, dLdG, kahan: bool = False):
+def vanilla_vjp(G, c, dLdG, kahan: bool = False, threads: bool = False):
     """Oracle for ``vanilla_vjp_numba(G, c, dLdG)`` (gradients.py:25-82).
 
     ``kahan=True`` sums the same terms with compensation instead of the
-    reference's left-to-right order.
+    reference's left-to-right order; with ``threads=True`` as well, the
+    compensated sums run over OpenMP threads (chunk sums added with
+    compensation, for large lattices).
     """
     G, g, c = _c128(G), _c128(dLdG), _c128(c).reshape(())
     D = G.ndim
@@ -114,7 +116,7 @@ def vanilla_vjp(G, c, dLdG, kahan: bool = False):
     db = np.zeros(D, np.complex128)
     dc = np.zeros((), np.complex128)
     s = _shape_arr(G.shape)
-    f = lib().oracle_vjp_kahan if kahan else lib().oracle_vjp
+    f = (lib().oracle_vjp_kahan_par if threads else lib().oracle_vjp_kahan) if kahan else lib().oracle_vjp
     _check(f(s.ctypes.data_as(_i64p), D, _ptr(G), _ptr(c), _ptr(g), _ptr(dA), _ptr(db), _ptr(dc)), "vjp")
     return dA, db, dc[()]
 

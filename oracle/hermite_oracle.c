@@ -336,6 +336,88 @@ int oracle_vjp_kahan(const int64_t *shape, int D, const double *G, const double 
     return vjp_impl(shape, D, G, c, g, dA, db, dc, 1);
 }
 
+/* The compensated sums of oracle_vjp_kahan split over OpenMP threads: the
+ * flat range is cut into fixed chunks, each chunk sums its terms with
+ * compensation, and the chunk sums are added with compensation in chunk
+ * order.  Deterministic for a given chunk count; same accuracy class as the
+ * serial compensated sum.  For lattices too large for a serial check. */
+int oracle_vjp_kahan_par(const int64_t *shape, int D, const double *G_, const double *c_,
+                         const double *g_, double *dA_out, double *db_out, double *dc_out) {
+    if (D < 1 || D > MAXD) return -1;
+    init_sqrt();
+    const cplx *G = (const cplx *)G_;
+    const cplx *g = (const cplx *)g_;
+    int64_t strides[MAXD];
+    strides_of(shape, D, strides);
+    const int64_t N = prod_of(shape, D);
+    const int NCH = 256;
+    const int NT = D * D + D + 1; /* dA (full D x D), db, sum G g */
+    double *part = (double *)calloc((size_t)NCH * NT * 2, sizeof(double));
+    if (!part) return -1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int ch = 0; ch < NCH; ++ch) {
+        const int64_t f0 = N * ch / NCH, f1 = N * (ch + 1) / NCH;
+        ksum acc[MAXD * MAXD + MAXD + 1][2];
+        memset(acc, 0, sizeof(ksum) * (size_t)NT * 2);
+        int64_t idx[MAXD];
+        int64_t rem = f0;
+        for (int a = D - 1; a >= 0; --a) {
+            idx[a] = rem % shape[a];
+            rem /= shape[a];
+        }
+        for (int64_t flat = f0; flat < f1; ++flat) {
+            const cplx gk = g[flat];
+            const cplx tg = c_mul(G[flat], gk);
+            kadd(&acc[NT - 1][0], tg.re);
+            kadd(&acc[NT - 1][1], tg.im);
+            if (flat > 0) {
+                for (int i = 0; i < D; ++i) {
+                    const int64_t pivot = flat - strides[i];
+                    const cplx tb = c_mul(s_mul(sq(idx[i]), rd(G, pivot, N)), gk);
+                    kadd(&acc[D * D + i][0], tb.re);
+                    kadd(&acc[D * D + i][1], tb.im);
+                    if (idx[i] > 1) {
+                        const double w = 0.5 * sq(idx[i]) * sq(idx[i] - 1);
+                        const cplx t = c_mul(s_mul(w, rd(G, pivot - strides[i], N)), gk);
+                        kadd(&acc[i * D + i][0], t.re);
+                        kadd(&acc[i * D + i][1], t.im);
+                    }
+                    for (int j = i + 1; j < D; ++j) {
+                        const double w = sq(idx[i]) * sq(idx[j]);
+                        const cplx t = c_mul(s_mul(w, rd(G, pivot - strides[j], N)), gk);
+                        kadd(&acc[i * D + j][0], t.re);
+                        kadd(&acc[i * D + j][1], t.im);
+                    }
+                }
+            }
+            next_index(idx, shape, D);
+        }
+        for (int e = 0; e < NT; ++e) {
+            part[((size_t)ch * NT + e) * 2 + 0] = acc[e][0].s + acc[e][0].c;
+            part[((size_t)ch * NT + e) * 2 + 1] = acc[e][1].s + acc[e][1].c;
+        }
+    }
+    cplx tot[MAXD * MAXD + MAXD + 1];
+    for (int e = 0; e < NT; ++e) {
+        ksum k[2];
+        memset(k, 0, sizeof(k));
+        for (int ch = 0; ch < NCH; ++ch) {
+            kadd(&k[0], part[((size_t)ch * NT + e) * 2 + 0]);
+            kadd(&k[1], part[((size_t)ch * NT + e) * 2 + 1]);
+        }
+        tot[e].re = k[0].s + k[0].c;
+        tot[e].im = k[1].s + k[1].c;
+    }
+    free(part);
+    cplx *dA_o = (cplx *)dA_out;
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) dA_o[i * D + j] = c_div_real(c_add(tot[i * D + j], tot[j * D + i]), 2.0);
+    memcpy(db_out, &tot[D * D], sizeof(cplx) * (size_t)D);
+    cplx dc = c_div(tot[NT - 1], *(const cplx *)c_);
+    memcpy(dc_out, &dc, sizeof(cplx));
+    return 0;
+}
+
 /* batch.py:54-61: prange over the batch (OpenMP here).  Every triple shares
  * one shape; G is (B, *shape), zero-initialised by the caller. */
 int oracle_vanilla_batch(const int64_t *shape, int D, int64_t B, const double *A,

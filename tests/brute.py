@@ -66,3 +66,46 @@ def hermite_series(A, b, c, shape, upper_only: bool = True) -> np.ndarray:
         v = cc * series.get(k, 0) * mpmath.sqrt(fact)
         G[k] = complex(v)
     return G
+
+
+def stable_tuple_recursion(A, b, c, shape) -> np.ndarray:
+    """The all-pivot average of stable_numba (core.py:164-213) written on index
+    tuples in mpmath at 50 digits: for every k != 0,
+        G_k = mean over i with k_i > 0 of
+              ( b_i G_{k-e_i} + sum_j A[i, j] sqrt(k_j - delta_ij) G_{k-e_i-e_j} ) / sqrt(k_i),
+    reading row i of A as given (both triangles).  Terms whose sqrt factor is
+    zero are left out (the reference multiplies them by SQRT[0] = 0).  No
+    strides, flat indices or operation order are shared with the oracle."""
+    A = np.asarray(A, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    D = len(shape)
+    mpc = lambda x: mpmath.mpc(complex(x).real, complex(x).imag)  # noqa: E731
+    Am = [[mpc(A[i, j]) for j in range(D)] for i in range(D)]
+    bm = [mpc(b[i]) for i in range(D)]
+    G = {tuple([0] * D): mpc(np.asarray(c).reshape(()))}
+
+    def get(k):
+        return G[k] if all(x >= 0 for x in k) else mpmath.mpc(0)
+
+    for k in itertools.product(*[range(s) for s in shape]):
+        if not any(k):
+            continue
+        acc, n = mpmath.mpc(0), 0
+        for i in range(D):
+            if k[i] == 0:
+                continue
+            n += 1
+            p = tuple(x - (a == i) for a, x in enumerate(k))
+            val = bm[i] * get(p)
+            for j in range(D):
+                w = k[j] - (1 if j == i else 0)
+                if w <= 0:
+                    continue
+                q = tuple(x - (a == j) for a, x in enumerate(p))
+                val += Am[i][j] * mpmath.sqrt(w) * get(q)
+            acc += val / mpmath.sqrt(k[i])
+        G[k] = acc / n
+    out = np.zeros(shape, dtype=np.complex128)
+    for k, v in G.items():
+        out[k] = complex(v)
+    return out

@@ -92,9 +92,11 @@ def test_contract_errors():
         contract(a, b, [0], [0, 1], [1])
     with pytest.raises(ValueError, match="Output labels"):
         contract(a, b, [0, 1], [1, 2], [3])
-    with pytest.raises(NotImplementedError):
-        contract(a, b, [0, 0], [0, 1], [1])  # a diagonal: einsum handles it, contract does not
-    # what einsum now takes: a diagonal and three operands
+    # a label repeated within an operand: that operand's diagonal, as numpy's einsum
+    assert relerr(contract(a, b, [0, 0], [0, 1], [1]).cpu().numpy(), np.einsum("aa,ab->b", a, b)) < TOL
+    with pytest.raises(ValueError, match="repeated axis"):
+        contract(rnd((3, 4), 3), b, [0, 0], [0, 1], [1])
+    # einsum: a diagonal and three operands
     assert relerr(einsum("ii,ij->j", a, b).cpu().numpy(), np.einsum("ii,ij->j", a, b)) < TOL
     assert relerr(einsum("ij,jk,kl->il", a, b, a).cpu().numpy(), a @ b @ a) < TOL
 
@@ -176,7 +178,53 @@ EINSUM_CASES = [
     ("...ab,...b->...a", [(2, 1, 3, 4), (1, 5, 4)]),
     ("ab,cd,bc->ad", [(3, 40), (50, 2), (40, 50)]),
     ("i,ij,jk,k->", [(6,), (6, 7), (7, 8), (8,)]),
+    ("i,i,j->j", [(5,), (5,), (4,)]),
+    (",ij->ij", [(), (3, 4)]),
+    ("ij,ij,k->k", [(3, 4), (3, 4), (6,)]),
 ]
+
+# size-1 axes of any label broadcast (numpy's rule), checked against numpy
+BROADCAST_CASES = [
+    ("ij,ij->ij", [(1, 3), (2, 3)]),
+    ("ij,jk->ik", [(2, 1), (3, 4)]),
+    ("ab,b->a", [(4, 1), (6,)]),
+]
+
+
+def _einsum_on_cpu(monkeypatch):
+    """einsum with the matrix-core product replaced by torch.matmul on the CPU,
+    so that its label, diagonal and broadcasting logic runs without a GPU."""
+    import torch
+
+    import mrmustard_amd.contraction as C
+
+    monkeypatch.setattr(C, "as_device_c128", lambda x: torch.as_tensor(x).to(torch.complex128))
+    monkeypatch.setattr(C, "_zgemm_batched", lambda A, B: torch.matmul(A, B))
+    return C
+
+
+@pytest.mark.parametrize("spec,shapes", EINSUM_CASES + BROADCAST_CASES, ids=lambda x: str(x))
+def test_einsum_logic_on_cpu(monkeypatch, spec, shapes):
+    C = _einsum_on_cpu(monkeypatch)
+    rng = np.random.default_rng(len(spec) + 100)
+    arrs = [rng.standard_normal(s) + 1j * rng.standard_normal(s) for s in shapes]
+    got = C.einsum(spec, *arrs).numpy()
+    ref = np.einsum(spec, *arrs)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_einsum_and_contract_errors_on_cpu(monkeypatch):
+    C = _einsum_on_cpu(monkeypatch)
+    with pytest.raises(ValueError, match="sizes"):
+        C.einsum("ij,jk->ik", np.zeros((2, 3)), np.zeros((4, 2)))
+    with pytest.raises(ValueError, match="repeated axis"):
+        C.einsum("ii->i", np.zeros((1, 3)))
+    a = np.arange(9.0).reshape(3, 3)
+    got = C.contract(a, np.ones((3, 2)), [0, 0], [0, 1], [1]).numpy()
+    assert np.allclose(got, np.einsum("aa,ab->b", a, np.ones((3, 2))))
+    with pytest.raises(ValueError, match="repeat"):
+        C.contract(a, a, [0, 1], [1, 2], [0, 0])
 
 
 @pytest.mark.gpu

@@ -147,6 +147,10 @@ def test_fock_array_derived_fused_window(n, shape_derived, shape):
     got = fock_array_derived(A, b, c, shape, fused=True).cpu().numpy()
     assert got.shape == (2,) + shape
     assert relerr(got, ref) < TOL
+    # an empty batch gives an empty array on both paths
+    A0, b0, c0 = (np.asarray(x)[:0] for x in (A, b, c))
+    for fu in (True, False):
+        assert tuple(fock_array_derived(A0, b0, c0, shape, fused=fu).shape) == (0,) + shape
 
 
 @pytest.mark.gpu

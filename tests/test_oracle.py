@@ -31,7 +31,7 @@ import numpy as np
 import pytest
 from scipy.special import eval_genlaguerre, eval_hermite, factorial
 
-from tests.brute import hermite_series
+from tests.brute import hermite_series, stable_tuple_recursion
 
 
 def rel(x, ref):
@@ -163,6 +163,25 @@ def test_stable_vs_power_series(oracle_mod, shape):
     assert rel(G, hermite_series(A, b, c, shape, upper_only=True)) < 1e-13
 
 
+@pytest.mark.parametrize("shape", [(2, 2), (3, 3), (4, 3), (2, 3, 2), (3, 2, 2, 2)])
+def test_stable_nonsymmetric_vs_tuple_recursion(oracle_mod, shape):
+    """On a non-symmetric A the all-pivot average reads both triangles, and the
+    pivots' recurrences disagree, so no generating function applies: the
+    oracle is checked against an independent tuple-indexed recursion in 50-digit
+    arithmetic (tests/brute.py).  A transposed A (A[j, i] read for A[i, j])
+    must be detected; on a (2, 2) box it cannot be, since the only point with
+    two pivots averages A[0, 1] and A[1, 0]."""
+    rng = np.random.default_rng(len(shape) * 31 + sum(shape))
+    A, b, c = random_triple(rng, len(shape), sym=False)
+    assert np.max(np.abs(A - A.T)) > 0.1
+    G = oracle_mod.stable(shape, A, b, c)
+    ref = stable_tuple_recursion(A, b, c, shape)
+    assert rel(G, ref) < 1e-13
+    if max(shape) > 2:
+        assert rel(stable_tuple_recursion(A.T, b, c, shape), ref) > 1e-6
+        assert rel(oracle_mod.stable(shape, A.T, b, c), ref) > 1e-6
+
+
 def test_box_independence(oracle_mod):
     rng = np.random.default_rng(5)
     A, b, c = random_triple(rng, 3)
@@ -278,3 +297,46 @@ def test_uneven_equals_loop(oracle_mod):
         assert np.array_equal(G, ref)
         rA, rb, rc = oracle_mod.vanilla_vjp(ref, c, g)
         assert np.array_equal(dA, rA) and np.array_equal(db, rb) and dc == rc
+
+
+# ------------------------------------------------------- displacement oracle
+
+
+def _displacement_closed_form(alpha, n, m):
+    """<n|D(alpha)|m> in 50-digit arithmetic from the generalized Laguerre
+    closed form (Cahill and Glauber, Phys. Rev. 177, 1857 (1969))."""
+    import mpmath
+
+    mpmath.mp.dps = 50
+    al = mpmath.mpc(alpha.real, alpha.imag)
+    x = abs(al) ** 2
+    if n >= m:
+        v = mpmath.sqrt(mpmath.factorial(m) / mpmath.factorial(n)) * al ** (n - m) * mpmath.laguerre(m, n - m, x)
+    else:
+        v = mpmath.sqrt(mpmath.factorial(n) / mpmath.factorial(m)) * (-mpmath.conj(al)) ** (m - n) * mpmath.laguerre(n, m - n, x)
+    return complex(v * mpmath.exp(-x / 2))
+
+
+@pytest.mark.parametrize("alpha", [4.0 + 4.0j, 0.3 - 1.1j, -2.5 + 0.0j])
+@pytest.mark.parametrize("cutoffs", [(150, 150), (150, 90), (60, 150)])
+def test_displacement_vs_laguerre_closed_form(alpha, cutoffs):
+    from oracle.displacement import displacement
+
+    D = displacement(cutoffs, alpha)
+    assert D.shape == cutoffs
+    rows, cols = range(0, cutoffs[0], 3), range(1, cutoffs[1], 3)
+    ref = np.array([[_displacement_closed_form(alpha, n, m) for m in cols] for n in rows])
+    assert rel(D[::3, 1::3], ref) < 1e-12
+    # a conjugated phase (the sign of i (n - m) arg(alpha)) is detected
+    if alpha.imag != 0:
+        assert rel(displacement(cutoffs, np.conj(alpha))[::3, 1::3], ref) > 1e-3
+
+
+def test_displacement_unitary_block():
+    """Rows n < 100 of D(4 + 4i) at cutoff 1000 are orthonormal: the truncated
+    rows hold all of their weight, so this pins the large-index entries."""
+    from oracle.displacement import displacement
+
+    D = displacement((1000, 1000), 4.0 + 4.0j)
+    P = D[:100] @ D[:100].conj().T
+    assert np.max(np.abs(P - np.eye(100))) < 1e-12
