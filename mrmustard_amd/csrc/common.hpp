@@ -33,6 +33,12 @@ __device__ __forceinline__ double2 cscale_promoted(double2 a, double s) {
     return make_double2(a.x * s - a.y * 0.0, a.x * 0.0 + a.y * s);
 }
 
+// c ? a : b, one component at a time (a select of the whole struct can be
+// lowered through private memory).
+__device__ __forceinline__ double2 csel(bool c, double2 a, double2 b) {
+    return make_double2(c ? a.x : b.x, c ? a.y : b.y);
+}
+
 // Fused complex multiply-accumulate acc += a*b (VJP reductions only).
 __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 acc) {
     acc.x = fma(a.x, b.x, acc.x);

@@ -244,10 +244,7 @@ __device__ __forceinline__ void level0_reg(const Job &J, const Team<WAVE> &tm, d
         p1v[r] = ring[q[r]];
         p2v[r] = zero;
         gcur[r] = zero;
-        if (FUSED && s0 > 1) {
-            const double2 t = g[int64_t(S) + q[r]];
-            gcur[r] = act[r] ? t : zero;
-        }
+        if (FUSED && s0 > 1) gcur[r] = g[int64_t(S) + q[r]];  // masked where used
         gnext[r] = zero;
     }
     for (int m = 1; m < s0; ++m) {
@@ -261,8 +258,7 @@ __device__ __forceinline__ void level0_reg(const Job &J, const Team<WAVE> &tm, d
 #pragma unroll
             for (int r = 0; r < RM; ++r) {
                 if (r >= nr) break;
-                const double2 t = g[gbase + S + q[r]];
-                gnext[r] = act[r] ? t : zero;
+                gnext[r] = g[gbase + S + q[r]];  // masked where used, so nothing waits for it here
             }
         }
         double2 vnew[RM];
@@ -294,7 +290,7 @@ __device__ __forceinline__ void level0_reg(const Job &J, const Team<WAVE> &tm, d
 #pragma unroll
             for (int r = 0; r < RM; ++r) {
                 if (r >= nr) break;
-                const double2 gk = gcur[r];
+                const double2 gk = csel(act[r], gcur[r], zero);
                 acc.c = cfma(vnew[r], gk, acc.c);
                 acc.b[0] = wacc(acc.b[0], sm, p1v[r], gk);
                 acc.a[0] = wacc(acc.a[0], w00, p2v[r], gk);
@@ -377,10 +373,7 @@ __device__ __forceinline__ void level_straight(const Job &J, const Team<WAVE> &t
         p2v[r] = zero;
         gcur[r] = zero;
         gnext[r] = zero;
-        if (VJP && sd > 1) {
-            const double2 t = g[int64_t(S) + q[r]];
-            gcur[r] = act[r] ? t : zero;
-        }
+        if (VJP && sd > 1) gcur[r] = g[int64_t(S) + q[r]];  // masked where used
     }
     for (int m = 1; m < sd; ++m) {
         double2 *cur = TOP ? ring + (m & 1) * S : ring + m * Sd;
@@ -392,8 +385,7 @@ __device__ __forceinline__ void level_straight(const Job &J, const Team<WAVE> &t
         if (VJP && m + 1 < sd) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const double2 t = g[gbase + Sd + q[r]];
-                gnext[r] = act[r] ? t : zero;
+                gnext[r] = g[gbase + Sd + q[r]];  // masked where used, so nothing waits for it here
             }
         }
         double2 v[NR];
@@ -428,7 +420,7 @@ __device__ __forceinline__ void level_straight(const Job &J, const Team<WAVE> &t
             const double h00 = 0.5 * sm1;
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const double2 gk = gcur[r];
+                const double2 gk = csel(act[r], gcur[r], zero);
                 u0[r] = make_double2(sm * gk.x, sm * gk.y);
                 acc.c = cfma(v[r], gk, acc.c);
                 acc.b[0] = cfma(p1v[r], u0[r], acc.b[0]);
@@ -445,7 +437,7 @@ __device__ __forceinline__ void level_straight(const Job &J, const Team<WAVE> &t
         if (VJP) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const double2 gk = gcur[r];
+                const double2 gk = csel(act[r], gcur[r], zero);
 #pragma unroll
                 for (int i = 1; i < D; ++i) {
                     const int ki = kd[r][i];

@@ -59,8 +59,9 @@ def _contract_derived(G, c, batch_shape, head_shape):
     """sum_k G[..., head, k] c[..., k] with the derived axes flattened into k."""
     nb = pymath.prod(batch_shape) if batch_shape else 1
     rows = pymath.prod(head_shape) if head_shape else 1
-    G3 = G.reshape(nb, rows, -1)
-    c3 = c.reshape(nb, -1, 1)
+    K = pymath.prod(c.shape[len(batch_shape):])
+    G3 = G.reshape(nb, rows, K)
+    c3 = c.reshape(nb, K, 1)
     return zgemm_batched_hip(G3, c3).reshape(tuple(batch_shape) + tuple(head_shape))
 
 
