@@ -82,6 +82,22 @@ int mm_hermite_batch_vjp(const double *G, const double *c, const double *dLdG, i
                          const int64_t *shape, int D, double *dA, double *db, double *dc,
                          void *workspace, size_t ws_bytes, void *stream);
 
+/* mm_hermite_batch_vjp without a workspace argument and without any device
+ * allocation: the partial sums live in a static device buffer of
+ * MM_STATIC_SCRATCH_BYTES per GPU, held until the process ends.  No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops).  mm_init(device) must have run on
+ * the current device first (it looks up the buffer's address); returns
+ * MM_ERR_UNSUPPORTED when B triples of this shape need more scratch than the
+ * buffer holds (mm_hermite_batch_vjp_workspace(B, shape, D) > the maximum).
+ * Every call on one GPU shares that GPU's buffer: calls that overlap on one
+ * GPU (different streams or threads) race and must be ordered by the caller.
+ * Results equal mm_hermite_batch_vjp's bit for bit. */
+#define MM_STATIC_SCRATCH_BYTES (8 << 20)
+int mm_hermite_batch_vjp_static(const double *G, const double *c, const double *dLdG, int64_t B,
+                                const int64_t *shape, int D, double *dA, double *db, double *dc,
+                                void *stream);
+
 /* Fused forward + VJP in one sweep when dL/dG is known up front: the
  * composition vanilla_numba -> vanilla_vjp_numba (core.py:25, gradients.py:25). */
 size_t mm_hermite_fused_workspace(const int64_t *shape, int D);

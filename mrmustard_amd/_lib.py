@@ -38,6 +38,7 @@ SIGNATURES = {
     "mm_hermite_vjp": (_int, [_vp, _vp, _vp, _vp, _int, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mm_hermite_batch_vjp_workspace": (_sz, [_i64, _vp, _int]),
     "mm_hermite_batch_vjp": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mm_hermite_batch_vjp_static": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _vp, _vp, _vp, _vp]),
     "mm_hermite_fused_workspace": (_sz, [_vp, _int]),
     "mm_hermite_fused": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mm_hermite_batch_fused_workspace": (_sz, [_i64, _vp, _int]),

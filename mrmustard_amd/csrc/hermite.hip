@@ -16,7 +16,41 @@
 
 using namespace mm;
 
+// Device scratch of mm_hermite_batch_vjp_static: the VJP's per-block partial
+// sums when the caller cannot allocate a workspace.  One copy per GPU, held
+// until the process ends.  Host code never uses this name as an address: the
+// address on each device comes from hipGetSymbolAddress (scratch_address).
+static __device__ __align__(256) char scratch[MM_STATIC_SCRATCH_BYTES];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch;  // device -> its copy of `scratch`
+
+// Look up `scratch` on the current device once (mm_init does it).
+int scratch_init() {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch.count(dev)) return MM_OK;
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(MM_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch.emplace(dev, p);
+    return MM_OK;
+}
+
+// Address of `scratch` on the current device, or an error if mm_init has not
+// run there.
+int scratch_address(void **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(dev);
+    if (it == g_scratch.end()) return fail(MM_ERR_INVALID, "mm_init(%d) has not been called on this device", dev);
+    *out = it->second;
+    return MM_OK;
+}
 
 // ------------------------------------------------------------------ geometry
 
@@ -727,6 +761,7 @@ int mm_init(int device) {
     if (device != cur) HIP_TRY(hipSetDevice(device));
     DevState *ds = nullptr;
     int rc = device_state(&ds);
+    if (rc == MM_OK) rc = scratch_init();
     if (device != cur) HIP_TRY(hipSetDevice(cur));
     return rc;
 }
@@ -802,6 +837,29 @@ int mm_hermite_batch_vjp(const double *G, const double *c, const double *dLdG, i
     rc = device_state(&ds);
     if (rc) return rc;
     return launch_vjp(S, B, G, dLdG, c, dA, db, dc, workspace, ws_bytes, ds, reinterpret_cast<hipStream_t>(stream));
+}
+
+int mm_hermite_batch_vjp_static(const double *G, const double *c, const double *dLdG, int64_t B, const int64_t *shape, int D,
+                                double *dA, double *db, double *dc, void *stream) {
+    Shape S;
+    int rc = make_shape(shape, D, &S);
+    if (rc) return rc;
+    if (B < 0) return fail(MM_ERR_INVALID, "negative batch size");
+    if (B == 0) return MM_OK;
+    if (B > 65535) return fail(MM_ERR_UNSUPPORTED, "batch of %lld > 65535 triples", (long long)B);
+    if (!G || !c || !dLdG || !dA || !db || !dc) return fail(MM_ERR_INVALID, "NULL array pointer");
+    const size_t need = vjp_ws_bytes(S, B);
+    if (need > size_t(MM_STATIC_SCRATCH_BYTES))
+        return fail(MM_ERR_UNSUPPORTED, "this VJP needs %zu bytes of scratch, above the static buffer's %zu; use mm_hermite_batch_vjp",
+                    need, size_t(MM_STATIC_SCRATCH_BYTES));
+    void *ws = nullptr;
+    rc = scratch_address(&ws);
+    if (rc) return rc;
+    // the device state exists: mm_init created it with the scratch address
+    DevState *ds = nullptr;
+    rc = device_state(&ds);
+    if (rc) return rc;
+    return launch_vjp(S, B, G, dLdG, c, dA, db, dc, ws, size_t(MM_STATIC_SCRATCH_BYTES), ds, reinterpret_cast<hipStream_t>(stream));
 }
 
 size_t mm_hermite_fused_workspace(const int64_t *shape, int D) {
