@@ -98,12 +98,13 @@ struct Slab0Args {
     int m;
 };
 
+// Block `blk` of `nblk` (256 threads each) over the slab's points.
 template <int D>
-__global__ __launch_bounds__(256) void fwd_slab0_kernel(Slab0Args a) {
+__device__ __forceinline__ void fwd_slab0_body(const Slab0Args &a, int64_t blk, int64_t nblk) {
     const int m = a.m;
     const int64_t S = a.g.stride[0];
-    const int64_t q0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t inc = int64_t(gridDim.x) * blockDim.x;
+    const int64_t q0 = blk * 256 + threadIdx.x;
+    const int64_t inc = nblk * 256;
     if (q0 >= S) return;
     int k[D], ki[D];
     digits_of<D>(q0, a.g, 1, k);
@@ -123,6 +124,11 @@ __global__ __launch_bounds__(256) void fwd_slab0_kernel(Slab0Args a) {
         a.cur[q] = cdiv_by(v, sm, rm);
         if (q + inc < S) digits_add<D>(k, ki, a.g, 1);
     }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void fwd_slab0_kernel(Slab0Args a) {
+    fwd_slab0_body<D>(a, blockIdx.x, gridDim.x);
 }
 
 __global__ void zero_c128_kernel(double2 *p, int64_t n) {
@@ -253,8 +259,9 @@ struct SlabVjpArgs {
     int m;
 };
 
+// Block `blk` of `nblk` (256 threads each) adds its sums into partial row blk.
 template <int D, int R0, int R1>
-__global__ __launch_bounds__(256) void vjp_slab_kernel(SlabVjpArgs a) {
+__device__ __forceinline__ void vjp_slab_body(const SlabVjpArgs &a, int64_t blk, int64_t nblk) {
     using RG = RowGroup<D, R0, R1>;
     constexpr int NL = RG::NL;
     const int m = a.m;
@@ -262,8 +269,8 @@ __global__ __launch_bounds__(256) void vjp_slab_kernel(SlabVjpArgs a) {
     double2 acc[NL];
 #pragma unroll
     for (int e = 0; e < NL; ++e) acc[e] = make_double2(0.0, 0.0);
-    const int64_t q0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t inc = int64_t(gridDim.x) * blockDim.x;
+    const int64_t q0 = blk * 256 + threadIdx.x;
+    const int64_t inc = nblk * 256;
     if (q0 < S) {
         int64_t st[D];
 #pragma unroll
@@ -313,7 +320,12 @@ __global__ __launch_bounds__(256) void vjp_slab_kernel(SlabVjpArgs a) {
             if (q + inc < S) digits_add<D>(k, ki, a.g, 1);
         }
     }
-    block_store_group<D, R0, R1, true>(acc, a.partial + int64_t(blockIdx.x) * RG::NA);
+    block_store_group<D, R0, R1, true>(acc, a.partial + blk * RG::NA);
+}
+
+template <int D, int R0, int R1>
+__global__ __launch_bounds__(256) void vjp_slab_kernel(SlabVjpArgs a) {
+    vjp_slab_body<D, R0, R1>(a, blockIdx.x, gridDim.x);
 }
 
 // Row groups per D: one for D <= 4, two for D = 5, 6, three for D = 7, 8.
@@ -323,6 +335,29 @@ struct RowSplit {
     static constexpr int B1 = D <= 4 ? D : 2;
     static constexpr int B2 = D <= 6 ? D : 4;
 };
+
+// One level-0 slab per launch, all of its work in that launch: the VJP terms
+// of slab m in every pivot-row group and (FWD) the forward of slab m + 1 from
+// slabs m and m - 1.  Block b has role b % NR (a row group, or the forward)
+// and covers chunk b / NR of the slab, so the NR blocks of one chunk run at
+// the same time and read the same slab data: each slab is fetched from HBM
+// about once per launch instead of once per row group.  The forward writes
+// slab m + 1, which no VJP term of this launch reads.
+template <int D, bool FWD>
+__global__ __launch_bounds__(256) void slab_fused_kernel(SlabVjpArgs v, Slab0Args f) {
+    using RS = RowSplit<D>;
+    constexpr int NR = RS::NG + (FWD ? 1 : 0);
+    const int64_t role = blockIdx.x % NR, blk = blockIdx.x / NR, nblk = gridDim.x / NR;
+    if (role == 0) {
+        vjp_slab_body<D, 0, RS::B1>(v, blk, nblk);
+    } else if (RS::NG >= 2 && role == 1) {
+        if constexpr (RS::NG >= 2) vjp_slab_body<D, RS::B1, RS::B2>(v, blk, nblk);
+    } else if (RS::NG >= 3 && role == 2) {
+        if constexpr (RS::NG >= 3) vjp_slab_body<D, RS::B2, D>(v, blk, nblk);
+    } else if (FWD) {
+        fwd_slab0_body<D>(f, blk, nblk);
+    }
+}
 
 // One block per triple.  npart partial rows of NA sums each (the VJP kernel's
 // chunks, optionally followed by extra rows from other kernels).  Wave w sums

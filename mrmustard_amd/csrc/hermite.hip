@@ -272,14 +272,6 @@ void launch_vjp_groups(const VjpArgs &a, dim3 grid, hipStream_t s) {
     if constexpr (RS::NG >= 3) hipLaunchKernelGGL((vjp_kernel<D, RS::B2, D>), grid, dim3(256), 0, s, a);
 }
 
-template <int D>
-void launch_vjp_slab_groups(const SlabVjpArgs &v, int nblk, hipStream_t s) {
-    using RS = RowSplit<D>;
-    hipLaunchKernelGGL((vjp_slab_kernel<D, 0, RS::B1>), dim3(unsigned(nblk)), dim3(256), 0, s, v);
-    if constexpr (RS::NG >= 2) hipLaunchKernelGGL((vjp_slab_kernel<D, RS::B1, RS::B2>), dim3(unsigned(nblk)), dim3(256), 0, s, v);
-    if constexpr (RS::NG >= 3) hipLaunchKernelGGL((vjp_slab_kernel<D, RS::B2, D>), dim3(unsigned(nblk)), dim3(256), 0, s, v);
-}
-
 // Threads of the finalize block of each triple: one wave per sum column for
 // few triples, four waves when many triples fill the GPU anyway.
 int finalize_threads(int64_t B) { return B >= 64 ? 256 : 1024; }
@@ -295,10 +287,19 @@ size_t vjp_ws_bytes(const Shape &S, int64_t B) {
     return size_t(B) * vjp_chunks(S.N, B) * nacc_of(S.D) * 16 + 256;
 }
 
+// VJP of one stored lattice slab by slab (all row groups of a slab in one
+// launch, so each slab is fetched about once rather than once per group):
+// taken for D >= 5 (two or three row groups) when the slabs are large.
+bool vjp_by_slabs(const Shape &S, int64_t B) { return B == 1 && S.D >= 5 && S.st[0] >= (int64_t(1) << 16); }
+
+int launch_vjp_slabs(const Shape &S, const double *G, const double *g, const double *c, double *dA, double *db, double *dc, void *ws,
+                     const DevState *ds, hipStream_t s);
+
 int launch_vjp(const Shape &S, int64_t B, const double *G, const double *g, const double *c, double *dA, double *db, double *dc,
                void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s) {
     if (!ws || ws_bytes < vjp_ws_bytes(S, B))
         return fail(MM_ERR_WORKSPACE, "VJP workspace of %zu bytes needed, %zu given", vjp_ws_bytes(S, B), ws_bytes);
+    if (vjp_by_slabs(S, B)) return launch_vjp_slabs(S, G, g, c, dA, db, dc, ws, ds, s);
     VjpArgs a;
     a.g = geom_of(S);
     a.G = reinterpret_cast<const double2 *>(G);
@@ -322,6 +323,72 @@ int launch_vjp(const Shape &S, int64_t B, const double *G, const double *g, cons
         default:
             return fail(MM_ERR_INVALID, "bad D");
     }
+    return MM_OK;
+}
+
+// One launch of slab_fused_kernel for level-0 slab m: the VJP terms of slab m
+// (cur, p1, p2 = slabs m, m-1, m-2) added into nblk partial rows, and, when
+// nxt is not NULL, the forward of slab m + 1 into nxt.
+template <int D>
+void launch_slab_fused_t(const SlabVjpArgs &v, const Slab0Args &f, bool fwd, int nblk, hipStream_t s) {
+    constexpr int NG = RowSplit<D>::NG;
+    if (fwd)
+        hipLaunchKernelGGL((slab_fused_kernel<D, true>), dim3(unsigned(nblk * (NG + 1))), dim3(256), 0, s, v, f);
+    else
+        hipLaunchKernelGGL((slab_fused_kernel<D, false>), dim3(unsigned(nblk * NG)), dim3(256), 0, s, v, f);
+}
+
+int launch_slab_fused(const Shape &S, const double *A, const double *b, const double2 *cur, const double2 *p1, const double2 *p2,
+                      const double2 *gg, double2 *nxt, double2 *partial, int nblk, int m, const DevState *ds, hipStream_t s) {
+    SlabVjpArgs v;
+    v.g = geom_of(S);
+    v.cur = cur;
+    v.p1 = p1;
+    v.p2 = p2;
+    v.gg = gg;
+    v.partial = partial;
+    v.sq = ds->sq;
+    v.m = m;
+    Slab0Args f;
+    f.g = v.g;
+    f.A = reinterpret_cast<const double2 *>(A);
+    f.b = reinterpret_cast<const double2 *>(b);
+    f.p1 = cur;
+    f.p2 = p1;
+    f.cur = nxt;
+    f.sq = ds->sq;
+    f.rsq = ds->rsq;
+    f.m = m + 1;
+    switch (S.D) {
+#define CASE(DD) \
+    case DD: launch_slab_fused_t<DD>(v, f, nxt != nullptr, nblk, s); break;
+        CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad D");
+    }
+    LAUNCH_CHECK("slab_fused_kernel");
+    return MM_OK;
+}
+
+// Blocks per row group of a slab_fused_kernel launch over slabs of S0 points.
+int slab_blocks(int64_t S0, const DevState *ds) {
+    return int(std::max<int64_t>(1, std::min<int64_t>((S0 + 1023) / 1024, int64_t(ds->num_cu) * 8)));
+}
+
+// Sum nblk partial rows of one triple and finish (symmetrise, dc = sum / c).
+int launch_finalize(int D, const double2 *partial, int nblk, const double *c, double *dA, double *db, double *dc, hipStream_t s) {
+    const double2 *c2 = reinterpret_cast<const double2 *>(c);
+    double2 *dA2 = reinterpret_cast<double2 *>(dA), *db2 = reinterpret_cast<double2 *>(db), *dc2 = reinterpret_cast<double2 *>(dc);
+    switch (D) {
+#define CASE(DD) \
+    case DD: hipLaunchKernelGGL((vjp_finalize<DD>), dim3(1), dim3(finalize_threads(1)), 0, s, partial, nblk, c2, dA2, db2, dc2); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad D");
+    }
+    LAUNCH_CHECK("vjp_finalize");
     return MM_OK;
 }
 
@@ -389,6 +456,25 @@ int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, c
     return MM_OK;
 }
 
+// The stored lattice's VJP, one slab_fused_kernel launch per level-0 slab
+// (no forward role).  The workspace holds at least slab_blocks rows.
+int launch_vjp_slabs(const Shape &S, const double *G, const double *g, const double *c, double *dA, double *db, double *dc, void *ws,
+                     const DevState *ds, hipStream_t s) {
+    const int64_t S0 = S.st[0];
+    const int nblk = std::min(slab_blocks(S0, ds), vjp_chunks(S.N, 1));
+    double2 *partial = reinterpret_cast<double2 *>(ws);
+    hipLaunchKernelGGL(zero_c128_kernel, dim3(64), dim3(256), 0, s, partial, int64_t(nblk) * nacc_of(S.D));
+    LAUNCH_CHECK("zero_c128_kernel");
+    const double2 *G2 = reinterpret_cast<const double2 *>(G), *g2 = reinterpret_cast<const double2 *>(g);
+    for (int m = 0; m < S.s[0]; ++m) {
+        const double2 *cur = G2 + int64_t(m) * S0;
+        const int rc = launch_slab_fused(S, nullptr, nullptr, cur, m >= 1 ? cur - S0 : cur, m >= 2 ? cur - 2 * S0 : cur, g2 + int64_t(m) * S0,
+                                         nullptr, partial, nblk, m, ds, s);
+        if (rc) return rc;
+    }
+    return launch_finalize(S.D, partial, nblk, c, dA, db, dc, s);
+}
+
 // ------------------------------------------------------------ vanilla paths
 
 // Workspace of one vanilla forward (+ fused VJP when `fused`).
@@ -399,6 +485,9 @@ size_t fwd_ws_bytes(const Shape &S, bool fused, const DevState *ds) {
     if (S.D == 1 || top == 0) return 0;
     return vjp_ws_bytes(S, 1);
 }
+
+int window_slab0(const Shape &S, const double *A, const double *b, double2 *cur, double2 *p1, double2 *p2, int m,
+                 const DevState *ds, hipStream_t s);
 
 // Forward of B triples sharing shape S (uniform strides); fused adds the VJP.
 int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b, const double *c, const double *g, double *G,
@@ -457,13 +546,37 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
         sa.G = reinterpret_cast<double2 *>(G) + t * a.ug;
         sa.sq = ds->sq;
         sa.rsq = ds->rsq;
-        for (int d = top - 1; d >= 0; --d) {
+        // with the VJP of a single triple, level 0 alternates the VJP terms of
+        // slab m (every row group in one launch) with the forward of slab m + 1,
+        // so that each slab is read while it is still in the caches
+        const bool fuse_level0 = fused && B == 1;
+        for (int d = top - 1; d >= (fuse_level0 ? 1 : 0); --d) {
             for (int m = 1; m < S.s[d]; ++m) {
                 sa.d = d;
                 sa.m = m;
                 rc = launch_slab(sa, S.D, S.st[d], ds->num_cu, s);
                 if (rc) return rc;
             }
+        }
+        if (fuse_level0) {
+            const int64_t S0 = S.st[0];
+            const int nblk = std::min(slab_blocks(S0, ds), vjp_chunks(S.N, 1));
+            double2 *partial = reinterpret_cast<double2 *>(ws);
+            hipLaunchKernelGGL(zero_c128_kernel, dim3(64), dim3(256), 0, s, partial, int64_t(nblk) * nacc_of(S.D));
+            LAUNCH_CHECK("zero_c128_kernel");
+            double2 *G2 = sa.G;
+            const double2 *g2 = reinterpret_cast<const double2 *>(g) + t * a.ug;
+            for (int m = 0; m < S.s[0]; ++m) {
+                double2 *cur = G2 + int64_t(m) * S0;
+                rc = launch_slab_fused(S, A + 2 * t * a.ua, b + 2 * t * a.ub, cur, m >= 1 ? cur - S0 : cur, m >= 2 ? cur - 2 * S0 : cur,
+                                       g2 + int64_t(m) * S0, nullptr, partial, nblk, m, ds, s);
+                if (rc) return rc;
+                if (m + 1 < S.s[0]) {
+                    rc = window_slab0(S, A + 2 * t * a.ua, b + 2 * t * a.ub, cur + S0, cur, m >= 1 ? cur - S0 : cur, m + 1, ds, s);
+                    if (rc) return rc;
+                }
+            }
+            return launch_finalize(S.D, partial, nblk, c + 2 * t, dA + 2 * t * a.ua, db + 2 * t * a.ub, dc + 2 * t, s);
         }
     }
     if (fused) return launch_vjp(S, B, G, g, c, dA, db, dc, ws, ws_bytes, ds, s);
@@ -479,10 +592,7 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
 // three slabs (3 * stride[0] * 16 B) instead of the lattice.  Lattices whose
 // slabs fit in LDS take the fused LDS sweep with a full G in the workspace.
 
-int recompute_blocks(const Shape &S, const DevState *ds) {
-    const int64_t S0 = S.st[0];
-    return int(std::max<int64_t>(1, std::min<int64_t>((S0 + 1023) / 1024, int64_t(ds->num_cu) * 8)));
-}
+int recompute_blocks(const Shape &S, const DevState *ds) { return slab_blocks(S.st[0], ds); }
 
 
 size_t recompute_ws_bytes(const Shape &S, const DevState *ds) {
@@ -569,10 +679,12 @@ int vjp_recompute(const Shape &S, const double *A, const double *b, const double
     if (S.D == 1 || topf == 0)
         return vanilla_uniform(S, 1, A, b, c, g, reinterpret_cast<double *>(ws), dA, db, dc, nullptr, 0, ds, s);
 
+    // three slabs: slab m + 1 is computed into the slot of slab m - 2 once the
+    // VJP terms of slab m (the last to read slab m - 2) are launched
     const size_t slab_bytes = align256(size_t(S.st[0]) * 16);
     char *base = reinterpret_cast<char *>(ws);
-    double2 *slot[3] = {reinterpret_cast<double2 *>(base), reinterpret_cast<double2 *>(base + slab_bytes),
-                        reinterpret_cast<double2 *>(base + 2 * slab_bytes)};
+    double2 *slot[3];
+    for (int i = 0; i < 3; ++i) slot[i] = reinterpret_cast<double2 *>(base + i * slab_bytes);
     double2 *partial = reinterpret_cast<double2 *>(base + 3 * slab_bytes);
     const int nblk = recompute_blocks(S, ds);
     const int NA = nacc_of(S.D);
@@ -582,45 +694,20 @@ int vjp_recompute(const Shape &S, const double *A, const double *b, const double
     // 1. the face {k_0 = 0} into slot 0
     int rc = window_face(S, A, b, c, slot[0], ds, s);
     if (rc) return rc;
-    // 2. level 0: VJP of slab 0, then recompute slab m and add its VJP terms
+    // 2. level 0: the VJP terms of slab m (every row group in one launch), then slab m + 1
     const int64_t S0 = S.st[0];
     for (int m = 0; m < S.s[0]; ++m) {
-        if (m >= 1) {
-            rc = window_slab0(S, A, b, slot[m % 3], slot[(m - 1) % 3], m >= 2 ? slot[(m - 2) % 3] : slot[(m - 1) % 3], m, ds, s);
+        double2 *cur = slot[m % 3];
+        rc = launch_slab_fused(S, A, b, cur, m >= 1 ? slot[(m + 2) % 3] : cur, m >= 2 ? slot[(m + 1) % 3] : cur,
+                               reinterpret_cast<const double2 *>(g) + int64_t(m) * S0, nullptr, partial, nblk, m, ds, s);
+        if (rc) return rc;
+        if (m + 1 < S.s[0]) {
+            rc = window_slab0(S, A, b, slot[(m + 1) % 3], cur, m >= 1 ? slot[(m + 2) % 3] : cur, m + 1, ds, s);
             if (rc) return rc;
         }
-        SlabVjpArgs v;
-        v.g = geom_of(S);
-        v.cur = slot[m % 3];
-        v.p1 = m >= 1 ? slot[(m - 1) % 3] : slot[0];
-        v.p2 = m >= 2 ? slot[(m - 2) % 3] : slot[0];
-        v.gg = reinterpret_cast<const double2 *>(g) + int64_t(m) * S0;
-        v.partial = partial;
-        v.sq = ds->sq;
-        v.m = m;
-        switch (S.D) {
-#define CASE(DD) \
-    case DD: launch_vjp_slab_groups<DD>(v, nblk, s); break;
-            CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-#undef CASE
-            default:
-                return fail(MM_ERR_INVALID, "bad D");
-        }
-        LAUNCH_CHECK("vjp_slab_kernel");
     }
     // 3. sum the block partials and finish (symmetrise, dc = sum / c)
-    const double2 *c2 = reinterpret_cast<const double2 *>(c);
-    double2 *dA2 = reinterpret_cast<double2 *>(dA), *db2 = reinterpret_cast<double2 *>(db), *dc2 = reinterpret_cast<double2 *>(dc);
-    switch (S.D) {
-#define CASE(DD) \
-    case DD: hipLaunchKernelGGL((vjp_finalize<DD>), dim3(1), dim3(finalize_threads(1)), 0, s, partial, nblk, c2, dA2, db2, dc2); break;
-        CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-#undef CASE
-        default:
-            return fail(MM_ERR_INVALID, "bad D");
-    }
-    LAUNCH_CHECK("vjp_finalize");
-    return MM_OK;
+    return launch_finalize(S.D, partial, nblk, c, dA, db, dc, s);
 }
 
 // ------------------------------------------------------------------- plans
