@@ -84,6 +84,49 @@ def traffic_from_profiles(name: str):
     return None
 
 
+def run_probe(which: str, dev) -> None:
+    """Run one workload three times without timing (kernel traces and counters)."""
+    import torch
+
+    from mrmustard_amd import workloads as W
+    from mrmustard_amd.lattice import strategies as st
+
+    if which == "headline":
+        (A, b, c), shape, g = W.batched_kets_30x3()
+        A, b, c, g = (torch.as_tensor(x, device=dev) for x in (A, b, c, g))
+        for _ in range(3):
+            st.vanilla_batch_fused_hip(shape, A, b, c, g)
+        n = int(np.prod(shape)) * A.shape[0]
+    elif which == "dm40":
+        (A, b, c), shape = W.lossy_dm_40x4()
+        A, b, c = (torch.as_tensor(np.asarray(x), device=dev) for x in (A, b, c))
+        for _ in range(3):
+            st.vanilla_hip(shape, A, b, c)
+        n = int(np.prod(shape))
+    elif which == "dm8":
+        (A, b, c), shape = W.dm8_10x8()
+        A, b, c = (torch.as_tensor(np.asarray(x), device=dev) for x in (A, b, c))
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(4)
+        g = torch.randn(shape, dtype=torch.complex128, device=dev, generator=gen)
+        G = st.vanilla_hip(shape, A, b, c)
+        st.vanilla_vjp_hip(G, c, g)
+        st.vanilla_vjp_recompute_hip(shape, A, b, c, g)
+        n = int(np.prod(shape))
+    else:
+        from mrmustard_amd.autotune import tuned_plan
+
+        triples, shapes = W.uneven_kets()
+        plan, _ = tuned_plan(shapes, fused=False)
+        Au, bu, cu = plan.pack(triples)
+        Gu = torch.empty(plan.n_amplitudes, dtype=torch.complex128, device=dev)
+        for _ in range(3):
+            plan.run(Au, bu, cu, G=Gu, graph=True)
+        n = plan.n_amplitudes
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": which, "amplitudes": n, "calls": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +149,12 @@ def main():
         action="store_true",
         help="only run the one-leftover-mode forward and its VJP on the 3-mode density matrix "
              "(20 x 20 x 15 x 15), three times each (for kernel traces)",
+    )
+    ap.add_argument(
+        "--probe",
+        choices=["headline", "dm40", "dm8", "uneven"],
+        help="only run one workload three times, untimed (for counter runs): the fused 256 x (30,30,30) batch, "
+             "the (40,)^4 forward, the (10,)^8 forward + VJP + recompute VJP, or the 4,096-ket plan",
     )
     args = ap.parse_args()
     if args.gpus != 1:
@@ -150,6 +199,9 @@ def main():
             leftover_vjp_hip(Alt, blt, clt, 19, (14, 14), cot)
         torch.cuda.synchronize()
         print(json.dumps({"leftover_probe_calls": 3, "vjp_calls": 3}))
+        return
+    if args.probe:
+        run_probe(args.probe, dev)
         return
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
 
