@@ -54,11 +54,11 @@ __device__ __forceinline__ double2 caxpy(double w, double2 z, double2 acc) {
 }
 
 // Numerators that Markstein's correction below cannot take: tiny non-zero
-// ones (the remainder would underflow) and infinite ones (x - s q is then
-// inf - inf).  They take the plain IEEE division, which keeps +-inf / s = +-inf.
+// ones (the remainder would underflow) and non-finite ones (x - s q is then
+// inf - inf).
 __device__ __forceinline__ bool div_rare(double x) {
     const double ax = fabs(x);
-    return ((ax < 0x1p-960) && (x != 0.0)) || (ax > 0x1.fffffffffffffp+1023);
+    return ((ax < 0x1p-960) && (x != 0.0)) || !(ax <= 0x1.fffffffffffffp+1023);
 }
 
 // Correctly rounded x / s given r = RN(1/s): Markstein's correction
@@ -73,8 +73,17 @@ __device__ __forceinline__ double div_by(double x, double s, double r) {
     return qq;
 }
 
+// v / s as numba divides a complex by a float: s is promoted to complex(s, 0)
+// and CPython's quotient gives ((x + y*0) / s, (y - x*0) / s).  For finite
+// parts that is x / s and y / s; an infinite part makes the other one NaN.
+__device__ __forceinline__ double2 cdiv_promoted(double2 v, double s) {
+    return make_double2((v.x + v.y * 0.0) / s, (v.y - v.x * 0.0) / s);
+}
+
 __device__ __forceinline__ double2 cdiv_by(double2 v, double s, double r) {
-    return make_double2(div_by(v.x, s, r), div_by(v.y, s, r));
+    double2 q = make_double2(div_by(v.x, s, r), div_by(v.y, s, r));
+    if (__builtin_expect(div_rare(v.x) || div_rare(v.y), 0)) q = cdiv_promoted(v, s);
+    return q;
 }
 
 // CPython's _Py_c_quot (numba's complex division), used for dL/dc = sum/c.
@@ -183,10 +192,8 @@ __device__ __forceinline__ void cdiv_many(double2 (&v)[N], double s, double r) {
     }
     if (__builtin_expect(tiny, 0)) {
 #pragma unroll
-        for (int e = 0; e < N; ++e) {
-            if (div_rare(v[e].x)) out[e].x = v[e].x / s;
-            if (div_rare(v[e].y)) out[e].y = v[e].y / s;
-        }
+        for (int e = 0; e < N; ++e)
+            if (div_rare(v[e].x) || div_rare(v[e].y)) out[e] = cdiv_promoted(v[e], s);
     }
 #pragma unroll
     for (int e = 0; e < N; ++e) v[e] = out[e];

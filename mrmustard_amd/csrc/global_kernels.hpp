@@ -4,6 +4,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "exact.hpp"
 #include "jobs.hpp"
 
 namespace mm {
@@ -52,6 +53,7 @@ struct SlabArgs {
     const double2 *A, *b;
     double2 *G;
     const double *sq, *rsq;
+    int *nf;  // raised when a stored value is not finite
     int d, m;
 };
 
@@ -73,6 +75,7 @@ __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
     for (int j = 0; j < D; ++j) Adj[j] = (j > d) ? a.A[d * D + j] : make_double2(0.0, 0.0);
     double2 *G = a.G;
     const int64_t base = int64_t(m) * Sd;
+    bool bad = false;
     for (int64_t q = q0; q < Sd; q += inc) {
         const int64_t f = base + q;
         double2 v = cmul(bd, G[f - Sd]);
@@ -82,9 +85,12 @@ __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
             if (j <= d) continue;
             if (k[j] > 0) v = cadd(v, cmul(cscale_promoted(Adj[j], a.sq[k[j]]), G[f - Sd - a.g.stride[j]]));
         }
-        G[f] = cdiv_by(v, sm, rm);
+        const double2 out = cdiv_by(v, sm, rm);
+        G[f] = out;
+        bad = bad || !finite2(out);
         if (q + inc < Sd) digits_add<D>(k, ki, a.g, d + 1);
     }
+    if (__builtin_expect(bad, 0) && a.nf) atomicOr(a.nf, 1);
 }
 
 // Level-0 slab m with the three slabs in separate buffers (a rolling window
@@ -493,6 +499,38 @@ __global__ __launch_bounds__(1024) void stable_block_kernel(StableArgs a) {
 }
 
 __global__ void set_first_kernel(double2 *G, const double2 *c) { G[0] = c[0]; }
+
+// ------------------------------------------------ non-finite values (exact.hpp)
+// Last launch of a multi-launch vanilla path: one workgroup per triple
+// reruns the triple in the reference's arithmetic when the path met a
+// non-finite value (the flag of a forward, or a non-finite dL/dc of a VJP,
+// whose sum of G * dL/dG every non-finite G or dL/dG value poisons).
+struct FixArgs {
+    Lat g;
+    const double2 *A, *b, *c, *gg;
+    double2 *G, *dA, *db, *dc;
+    const double *sq;
+    int64_t ua, ub, uc, ug;  // per-triple strides (complex elements)
+    const int *nf;           // forward: the path's flag, or NULL
+    int fwd, vjp;
+};
+
+__global__ __launch_bounds__(1024) void exact_fixup_kernel(FixArgs a) {
+    __shared__ double2 red[16];
+    __shared__ int s_bad;
+    const int64_t t = blockIdx.x;
+    if (threadIdx.x == 0) {
+        bool bad = a.nf && __hip_atomic_load(a.nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        if (a.vjp) bad = bad || !finite2(a.dc[t * a.uc]);
+        s_bad = bad;
+    }
+    __syncthreads();
+    if (!s_bad) return;
+    const BlockOps ops(red);
+    const double2 c = a.c[t * a.uc];
+    if (a.fwd) exact_forward(ops, a.g, 0, a.A + t * a.ua, a.b + t * a.ub, c, a.G + t * a.ug, a.sq);
+    if (a.vjp) exact_vjp(ops, a.g, a.G + t * a.ug, a.gg + t * a.ug, c, a.dA + t * a.ua, a.db + t * a.ub, a.dc + t * a.uc, a.sq);
+}
 
 __global__ void zero_ints_kernel(int *p, int n) {
     if (int(threadIdx.x) < n) p[threadIdx.x] = 0;

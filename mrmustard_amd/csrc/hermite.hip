@@ -22,6 +22,13 @@ using namespace mm;
 // address on each device comes from hipGetSymbolAddress (scratch_address).
 static __device__ __align__(256) char scratch[MM_STATIC_SCRATCH_BYTES];
 
+// Non-finite flags of the multi-launch forward paths, one per stream: calls
+// on one stream run in order, so no two calls in flight share a flag unless
+// more than NF_SLOTS streams are in use (then slots are reused).  Each call
+// clears its flag with its first launch and reads it with its last.
+constexpr int NF_SLOTS = 1024;
+static __device__ int nf_ring[NF_SLOTS];
+
 namespace {
 
 std::mutex g_scratch_mu;
@@ -37,6 +44,54 @@ int scratch_init() {
     HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
     if (!p) return fail(MM_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
     g_scratch.emplace(dev, p);
+    return MM_OK;
+}
+
+std::mutex g_nf_mu;
+std::map<int, int *> g_nf_base;                         // device -> its copy of nf_ring
+std::map<std::pair<int, hipStream_t>, int> g_nf_slot;  // (device, stream) -> slot
+int g_nf_next = 0;
+
+int nf_base(int dev, int **out) {
+    auto it = g_nf_base.find(dev);
+    if (it != g_nf_base.end()) {
+        *out = it->second;
+        return MM_OK;
+    }
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(nf_ring)));
+    if (!p) return fail(MM_ERR_HIP, "hipGetSymbolAddress(nf_ring) returned NULL");
+    g_nf_base.emplace(dev, static_cast<int *>(p));
+    *out = static_cast<int *>(p);
+    return MM_OK;
+}
+
+int nf_init() {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_nf_mu);
+    int *base = nullptr;
+    return nf_base(dev, &base);
+}
+
+// The flag of stream s on the current device.
+int nf_flag(hipStream_t s, int **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_nf_mu);
+    int *base = nullptr;
+    const int rc = nf_base(dev, &base);
+    if (rc) return rc;
+    const auto key = std::make_pair(dev, s);
+    auto it = g_nf_slot.find(key);
+    int slot = 0;
+    if (it == g_nf_slot.end()) {
+        slot = g_nf_next++ % NF_SLOTS;
+        g_nf_slot.emplace(key, slot);
+    } else {
+        slot = it->second;
+    }
+    *out = base + slot;
     return MM_OK;
 }
 
@@ -240,6 +295,7 @@ Bases bases(const double *A, const double *b, const double *c, const double *g, 
     P.db = reinterpret_cast<double2 *>(db);
     P.dc = reinterpret_cast<double2 *>(dc);
     P.partial = reinterpret_cast<double2 *>(partial);
+    P.nf = nullptr;
     return P;
 }
 
@@ -258,6 +314,51 @@ int launch_slab(const SlabArgs &a, int D, int64_t Sd, int num_cu, hipStream_t s)
             return fail(MM_ERR_INVALID, "bad D");
     }
     LAUNCH_CHECK("fwd_slab_kernel");
+    return MM_OK;
+}
+
+// ------------------------------------------------------ non-finite fix-up
+
+Lat lat_of(const Shape &S) {
+    Lat g;
+    memset(&g, 0, sizeof(g));
+    g.D = S.D;
+    for (int i = 0; i < 8; ++i) {
+        g.shape[i] = i < S.D ? S.s[i] : 1;
+        g.stride[i] = i < S.D ? S.st[i] : 1;
+    }
+    g.N = S.N;
+    return g;
+}
+
+// Last launch of a multi-launch path over B uniform triples: reruns a triple
+// in the reference's arithmetic when the path met a non-finite value (the
+// forward's flag nf, or a non-finite dc when the VJP was computed).
+int launch_fixup(const Shape &S, int64_t B, const double *A, const double *b, const double *c, const double *g, double *G, double *dA,
+                 double *db, double *dc, const int *nf, bool fwd, bool vjp, const DevState *ds, hipStream_t s) {
+    if (B < 1) return MM_OK;
+    if (B > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "too many triples for one launch");
+    FixArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = lat_of(S);
+    a.A = reinterpret_cast<const double2 *>(A);
+    a.b = reinterpret_cast<const double2 *>(b);
+    a.c = reinterpret_cast<const double2 *>(c);
+    a.gg = reinterpret_cast<const double2 *>(g);
+    a.G = reinterpret_cast<double2 *>(G);
+    a.dA = reinterpret_cast<double2 *>(dA);
+    a.db = reinterpret_cast<double2 *>(db);
+    a.dc = reinterpret_cast<double2 *>(dc);
+    a.sq = ds->sq;
+    a.ua = int64_t(S.D) * S.D;
+    a.ub = S.D;
+    a.uc = 1;
+    a.ug = S.N;
+    a.nf = nf;
+    a.fwd = fwd;
+    a.vjp = vjp;
+    hipLaunchKernelGGL(exact_fixup_kernel, dim3(unsigned(B)), dim3(1024), 0, s, a);
+    LAUNCH_CHECK("exact_fixup_kernel");
     return MM_OK;
 }
 
@@ -295,8 +396,20 @@ bool vjp_by_slabs(const Shape &S, int64_t B) { return B == 1 && S.D >= 5 && S.st
 int launch_vjp_slabs(const Shape &S, const double *G, const double *g, const double *c, double *dA, double *db, double *dc, void *ws,
                      const DevState *ds, hipStream_t s);
 
+int launch_vjp_sums(const Shape &S, int64_t B, const double *G, const double *g, const double *c, double *dA, double *db, double *dc,
+                    void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s);
+
+// The stored lattice's VJP, then the non-finite fix-up (fix = false when the
+// caller launches its own).
 int launch_vjp(const Shape &S, int64_t B, const double *G, const double *g, const double *c, double *dA, double *db, double *dc,
-               void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s) {
+               void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s, bool fix = true) {
+    int rc = launch_vjp_sums(S, B, G, g, c, dA, db, dc, ws, ws_bytes, ds, s);
+    if (rc || !fix) return rc;
+    return launch_fixup(S, B, nullptr, nullptr, c, g, const_cast<double *>(G), dA, db, dc, nullptr, false, true, ds, s);
+}
+
+int launch_vjp_sums(const Shape &S, int64_t B, const double *G, const double *g, const double *c, double *dA, double *db, double *dc,
+                    void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s) {
     if (!ws || ws_bytes < vjp_ws_bytes(S, B))
         return fail(MM_ERR_WORKSPACE, "VJP workspace of %zu bytes needed, %zu given", vjp_ws_bytes(S, B), ws_bytes);
     if (vjp_by_slabs(S, B)) return launch_vjp_slabs(S, G, g, c, dA, db, dc, ws, ds, s);
@@ -527,9 +640,16 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
     top = choose_top(S, false, ds->max_lds, lds_maxt(S.D, false), &T);
     if (fused && (!ws || ws_bytes < vjp_ws_bytes(S, B)))
         return fail(MM_ERR_WORKSPACE, "fused workspace of %zu bytes needed, %zu given", vjp_ws_bytes(S, B), ws_bytes);
+    int *nf = nullptr;
+    {
+        const int rc = nf_flag(s, &nf);
+        if (rc) return rc;
+    }
     for (int64_t t = 0; t < B; ++t) {
         LdsArgs at = a;
         at.P.g = nullptr;
+        at.P.nf = nf;
+        at.clear_nf = t == 0;  // the first launch of the path (one workgroup)
         at.n = 1;
         at.tmpl = make_job(S, top, KIND_BLOCK);
         at.tmpl.a_off = t * a.ua;
@@ -546,6 +666,7 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
         sa.G = reinterpret_cast<double2 *>(G) + t * a.ug;
         sa.sq = ds->sq;
         sa.rsq = ds->rsq;
+        sa.nf = nf;
         // with the VJP of a single triple, level 0 alternates the VJP terms of
         // slab m (every row group in one launch) with the forward of slab m + 1,
         // so that each slab is read while it is still in the caches
@@ -576,11 +697,17 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
                     if (rc) return rc;
                 }
             }
-            return launch_finalize(S.D, partial, nblk, c + 2 * t, dA + 2 * t * a.ua, db + 2 * t * a.ub, dc + 2 * t, s);
+            rc = launch_finalize(S.D, partial, nblk, c + 2 * t, dA + 2 * t * a.ua, db + 2 * t * a.ub, dc + 2 * t, s);
+            if (rc) return rc;
+            return launch_fixup(S, 1, A + 2 * t * a.ua, b + 2 * t * a.ub, c + 2 * t, g + 2 * t * a.ug, G + 2 * t * a.ug, dA + 2 * t * a.ua,
+                                db + 2 * t * a.ub, dc + 2 * t, nf, true, true, ds, s);
         }
     }
-    if (fused) return launch_vjp(S, B, G, g, c, dA, db, dc, ws, ws_bytes, ds, s);
-    return MM_OK;
+    if (fused) {
+        const int rc = launch_vjp(S, B, G, g, c, dA, db, dc, ws, ws_bytes, ds, s, false);
+        if (rc) return rc;
+    }
+    return launch_fixup(S, B, A, b, c, g, G, dA, db, dc, nf, true, fused, ds, s);
 }
 
 // ------------------------------------------------------- recompute VJP path
@@ -632,6 +759,7 @@ int window_face(const Shape &S, const double *A, const double *b, const double *
     sa.G = face;
     sa.sq = ds->sq;
     sa.rsq = ds->rsq;
+    sa.nf = nullptr;
     for (int d = top - 1; d >= 1; --d) {
         for (int m = 1; m < S.s[d]; ++m) {
             sa.d = d;
@@ -849,6 +977,7 @@ int mm_init(int device) {
     DevState *ds = nullptr;
     int rc = device_state(&ds);
     if (rc == MM_OK) rc = scratch_init();
+    if (rc == MM_OK) rc = nf_init();
     if (device != cur) HIP_TRY(hipSetDevice(cur));
     return rc;
 }

@@ -22,7 +22,8 @@ enum JobFlags : int32_t {
 // level-`top` slabs have S = stride[top] points.
 struct Job {
     int32_t D, kind, top, S;
-    int32_t flags, nacc, smax, pad;
+    int32_t flags, nacc, smax;
+    int32_t nf;  // set by the team that swept the job when it stored a non-finite value
     int32_t shape[8];
     int64_t stride[8];
     int32_t kbit[8];   // bit offset of digit k_j in the packed per-point index word

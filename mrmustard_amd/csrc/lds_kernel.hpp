@@ -23,6 +23,7 @@ struct LdsArgs {
     const Job *table;        // queue mode: block jobs, then wave jobs, then lane jobs
     int64_t nb, nw, nl;
     int *counters;           // queue mode: 3 zeroed counters
+    int clear_nf;            // one-workgroup launch that starts a multi-launch path: clear P.nf first
 };
 
 template <int DMAX, bool FUSED, bool WAVE>
@@ -65,6 +66,7 @@ __global__ __launch_bounds__(MAXT) void lds_kernel(LdsArgs a) {
     const int wpb = blockDim.x / MM_WAVE;
     const Bases P = a.P;
     const Tables T = a.T;
+    if (a.clear_nf && threadIdx.x == 0) *P.nf = 0;  // the sweep raises it only after barriers
 
     if (a.mode == MODE_UNIFORM) {
         JobShift sh;

@@ -19,6 +19,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "exact.hpp"
 #include "jobs.hpp"
 
 // Phase markers of the sweep (set up, face levels, face VJP, level 0,
@@ -32,7 +33,13 @@ namespace mm {
 struct Bases {
     const double2 *A, *b, *c, *g;
     double2 *G, *dA, *db, *dc, *partial;
+    int *nf;  // forward-only sweeps that are part of a longer path: raise this flag on a non-finite value (else the team recomputes)
 };
+
+// Note a non-finite amplitude in the team's job header (LDS).
+__device__ __forceinline__ void note_value(const Job &J, double2 v) {
+    if (__builtin_expect(!finite2(v), 0)) const_cast<Job &>(J).nf = 1;
+}
 
 struct Tables {
     const double *sq, *rsq;  // sqrt(n), RN(1/sqrt(n)), n < 100000
@@ -139,8 +146,10 @@ __device__ __forceinline__ void vjp_point(Acc<D> &acc, const Job &J, const doubl
 
 // Sum the accumulators over the team and write either the final gradients
 // (gradients.py:80-82: symmetrised dA, db, dc = sum(G*g)/c) or the raw sums.
+// Returns the team's sum of G * dL/dG (before the division by c) to every
+// thread of the team.
 template <int D, bool WAVE>
-__device__ void reduce_and_store(Acc<D> &acc, const Job &J, const Bases &P, const Team<WAVE> &tm, double2 *red, double2 c) {
+__device__ double2 reduce_and_store(Acc<D> &acc, const Job &J, const Bases &P, const Team<WAVE> &tm, double2 *red, double2 c) {
     constexpr int NA = D * (D + 1) / 2 + D + 1;
     double2 v[NA];
 #pragma unroll
@@ -164,17 +173,19 @@ __device__ void reduce_and_store(Acc<D> &acc, const Job &J, const Bases &P, cons
             red[threadIdx.x] = t;
         }
         __syncthreads();
-        if (threadIdx.x != 0) return;
+        const double2 total = red[NA - 1];
+        if (threadIdx.x != 0) return total;
 #pragma unroll
         for (int e = 0; e < NA; ++e) v[e] = red[e];
-    } else if (lane != MM_WAVE - 1) {
-        return;
+    } else {
+        const double2 total = make_double2(__shfl(v[NA - 1].x, MM_WAVE - 1, MM_WAVE), __shfl(v[NA - 1].y, MM_WAVE - 1, MM_WAVE));
+        if (lane != MM_WAVE - 1) return total;
     }
     if (J.flags & JOB_WRITE_PARTIAL) {
         double2 *out = P.partial + J.p_off;
 #pragma unroll
         for (int e = 0; e < NA; ++e) out[e] = v[e];
-        return;
+        return v[NA - 1];
     }
     double2 *dA = P.dA + J.a_off;
 #pragma unroll
@@ -190,6 +201,7 @@ __device__ void reduce_and_store(Acc<D> &acc, const Job &J, const Bases &P, cons
 #pragma unroll
     for (int i = 0; i < D; ++i) db[i] = v[D * (D + 1) / 2 + i];
     P.dc[J.c_off] = cdiv_py(v[NA - 1], c);
+    return v[NA - 1];
 }
 
 // Level 0 of a triple swept whole in LDS, with every slab point of the team
@@ -282,6 +294,7 @@ __device__ __forceinline__ void level0_reg(const Job &J, const Team<WAVE> &tm, d
             if (act[r]) {
                 cur[q[r]] = vnew[r];
                 G[gbase + q[r]] = vnew[r];
+                if (!FUSED) note_value(J, vnew[r]);
             }
         }
         tm.sync();
@@ -411,6 +424,7 @@ __device__ __forceinline__ void level_straight(const Job &J, const Team<WAVE> &t
             if (act[r]) {
                 cur[q[r]] = v[r];
                 G[gbase + q[r]] = v[r];
+                if (!VJP) note_value(J, v[r]);
             }
         }
         // VJP terms that need no other lane's slab-m value go before the
@@ -607,6 +621,7 @@ __device__ __forceinline__ void sweep_lds(const Bases P, const Tables T, char *l
     if (tm.rank == 0) {
         ring[0] = P.c[J.c_off];
         G[0] = P.c[J.c_off];
+        if (!FUSED) note_value(J, P.c[J.c_off]);
     }
     tm.sync();
 
@@ -641,6 +656,7 @@ __device__ __forceinline__ void sweep_lds(const Bases P, const Tables T, char *l
                         v = cdiv_by(v, sq[m], rsq[m]);
                         ring[m] = v;
                         G[m] = v;
+                        if (!FUSED) note_value(J, v);
                         m2 = m1;
                         m1 = v;
                     }
@@ -672,6 +688,7 @@ __device__ __forceinline__ void sweep_lds(const Bases P, const Tables T, char *l
                     v = cdiv_by(v, sm, rm);
                     ring[f] = v;
                     G[f] = v;
+                    if (!FUSED) note_value(J, v);
                 }
                 if (one_wave)
                     wave_sync();
@@ -720,6 +737,7 @@ __device__ __forceinline__ void sweep_lds(const Bases P, const Tables T, char *l
                 v = cdiv_by(v, sm, rm);
                 cur[q] = v;
                 G[gbase + q] = v;
+                if (!FUSED) note_value(J, v);
             }
             tm.sync();
             if (FUSED) {
@@ -729,9 +747,39 @@ __device__ __forceinline__ void sweep_lds(const Bases P, const Tables T, char *l
     }
 
     MM_PHASE(4);
-    if (FUSED) reduce_and_store<D, WAVE>(acc, J, P, tm, red, c);
+    double2 total = make_double2(0.0, 0.0);
+    if (FUSED) total = reduce_and_store<D, WAVE>(acc, J, P, tm, red, c);
     tm.sync();
     MM_PHASE(5);
+
+    // ---- non-finite values: rerun the triple in the reference's arithmetic
+    // (the fused sum of G * dL/dG is non-finite whenever G or dL/dG holds a
+    // non-finite value)
+    const bool bad = FUSED ? (!finite2(total) && top == 0 && !(J.flags & JOB_WRITE_PARTIAL)) : (J.nf != 0);
+    if (__builtin_expect(bad, 0)) {
+        if (!FUSED && P.nf) {
+            if (tm.rank == 0) atomicOr(P.nf, 1);
+        } else {
+            Lat lat;
+            lat.D = D;
+            for (int j = 0; j < 8; ++j) {
+                lat.shape[j] = J.shape[j];
+                lat.stride[j] = J.stride[j];
+            }
+            lat.N = int64_t(J.shape[0]) * J.stride[0];
+            const double2 *bg = P.b + J.b_off;
+            if (WAVE) {
+                const WaveOps ops;
+                exact_forward(ops, lat, top, Ag, bg, c, G, T.sq);
+                if (FUSED) exact_vjp(ops, lat, G, g, c, P.dA + J.a_off, P.db + J.b_off, P.dc + J.c_off, T.sq);
+            } else {
+                const BlockOps ops(red);
+                exact_forward(ops, lat, top, Ag, bg, c, G, T.sq);
+                if (FUSED) exact_vjp(ops, lat, G, g, c, P.dA + J.a_off, P.db + J.b_off, P.dc + J.c_off, T.sq);
+            }
+        }
+        tm.sync();
+    }
 }
 
 // One-index chain in a single lane (D = 1: core.py:108-122 with D = 1).
@@ -766,6 +814,15 @@ __device__ __forceinline__ void chain_lane(int s0, int64_t a_off, int64_t b_off,
         P.dA[a_off] = ab;
         P.db[b_off] = bb;
         P.dc[c_off] = cdiv_py(cc, c);
+        // the forward of one index has no zero-weight terms; the VJP has
+        if (__builtin_expect(!finite2(cc), 0)) {
+            Lat lat;
+            lat.D = 1;
+            lat.shape[0] = s0;
+            lat.stride[0] = 1;
+            lat.N = s0;
+            exact_vjp(LaneOps(), lat, G, g, c, P.dA + a_off, P.db + b_off, P.dc + c_off, T.sq);
+        }
     }
 }
 

@@ -109,3 +109,83 @@ def stable_tuple_recursion(A, b, c, shape) -> np.ndarray:
     for k, v in G.items():
         out[k] = complex(v)
     return out
+
+
+# ------------------------------------------------------------------------
+# CPython complex arithmetic on Python lists (non-finite values).
+#
+# numba compiles the reference's complex products and quotients to CPython's
+# own algorithms (_Py_c_prod, _Py_c_quot, a float promoted to complex(x, 0.0))
+# and wraps negative array indices as numpy does.  The two functions below
+# run the reference's loops (core.py:64-124, gradients.py:41-82) in plain
+# Python complex numbers on lists, where these rules are the interpreter's
+# own rather than the C oracle's restatement of them, so that the oracle's
+# non-finite results (0 * inf and inf - inf giving NaN, the NaN part of an
+# infinite number divided by a float) can be checked on small lattices.
+
+
+def _strides(shape):
+    st = [1] * len(shape)
+    for i in range(len(shape) - 1, 0, -1):
+        st[i - 1] = st[i] * shape[i]
+    return st
+
+
+def python_complex_vanilla(A, b, c, shape) -> np.ndarray:
+    A = [[complex(x) for x in row] for row in np.asarray(A, dtype=np.complex128)]
+    b = [complex(x) for x in np.asarray(b, dtype=np.complex128)]
+    D = len(shape)
+    st = _strides(shape)
+    N = int(np.prod(shape))
+    SQRT = [pymath.sqrt(n) for n in range(max(shape) + 2)]
+    SQRT_m1 = pymath.sqrt(99999)  # SQRT[-1] of the reference's table
+    sq = lambda n: SQRT[n] if n >= 0 else SQRT_m1  # noqa: E731
+    G = [0j] * N
+    G[0] = complex(c)
+    for flat, index in enumerate(itertools.product(*[range(s) for s in shape])):
+        if flat == 0:
+            continue
+        i = next(a for a in range(D) if flat - st[a] >= 0)
+        pivot = flat - st[i]
+        v = b[i] * G[pivot]
+        v += A[i][i] * sq(index[i] - 1) * G[pivot - st[i]]
+        for j in range(i + 1, D):
+            v += A[i][j] * sq(index[j]) * G[pivot - st[j]]
+        G[flat] = v / sq(index[i])
+    return np.array(G, dtype=np.complex128).reshape(shape)
+
+
+def python_complex_vjp(G, c, g):
+    shape = G.shape
+    D = len(shape)
+    st = _strides(shape)
+    Gl = [complex(x) for x in G.ravel()]
+    gl = [complex(x) for x in g.ravel()]
+    SQRT = [pymath.sqrt(n) for n in range(max(shape) + 2)]
+    N = len(Gl)
+    # an index below -N (a leading axis of size 1) makes numba read before
+    # the array, under a zero weight; it is taken modulo N here as the oracle does
+    Gw = lambda x: Gl[x % N]  # noqa: E731
+    dLdA = [[0j] * D for _ in range(D)]
+    dLdb = [0j] * D
+    dA = [[0j] * D for _ in range(D)]
+    db = [0j] * D
+    for flat, index in enumerate(itertools.product(*[range(s) for s in shape])):
+        if flat == 0:
+            continue
+        for i in range(D):
+            pivot = flat - st[i]
+            db[i] = SQRT[index[i]] * Gw(pivot)
+            dA[i][i] = 0.5 * SQRT[index[i]] * SQRT[index[i] - 1] * Gw(pivot - st[i]) if index[i] > 1 else 0j
+            for j in range(i + 1, D):
+                dA[i][j] = SQRT[index[i]] * SQRT[index[j]] * Gw(pivot - st[j])
+        for i in range(D):
+            for j in range(D):
+                dLdA[i][j] += dA[i][j] * gl[flat]
+            dLdb[i] += db[i] * gl[flat]
+    s = 0j
+    for x, y in zip(Gl, gl):
+        s += x * y
+    dc = s / complex(c)
+    out = np.array([[(dLdA[i][j] + dLdA[j][i]) / 2 for j in range(D)] for i in range(D)], dtype=np.complex128)
+    return out, np.array(dLdb, dtype=np.complex128), complex(dc)
