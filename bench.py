@@ -544,8 +544,9 @@ def main():
         shapes = [shape] * B
         gs = [g_np[k] for k in range(B)]
         reps, t0 = 0, time.perf_counter()
+        oracle.uneven(triples[:2], shapes[:2], gs[:2], host_tuned=True)  # build the host-tuned copy untimed
         while True:
-            oracle.uneven(triples, shapes, gs)
+            oracle.uneven(triples, shapes, gs, host_tuned=True)
             reps += 1
             if time.perf_counter() - t0 > 5.0:
                 break
@@ -556,7 +557,7 @@ def main():
             "cores": threads,
             "kind": "port",
             "sample": f"the full headline batch (256 x (30,30,30), forward + VJP) x {reps} repetitions, "
-            f"{wall:.1f} s wall, OpenMP over triples",
+            f"{wall:.1f} s wall, OpenMP over triples; gcc -O3 -march=native -ffp-contract=off built on this host",
         }
 
     out = {

@@ -41,25 +41,46 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        for name in ("oracle_vanilla", "oracle_stable"):
-            f = getattr(L, name)
-            f.argtypes = [_i64p, ctypes.c_int, _dp, _dp, _dp, _dp]
-            f.restype = ctypes.c_int
-        for name in ("oracle_vjp", "oracle_vjp_kahan", "oracle_vjp_kahan_par"):
-            f = getattr(L, name)
-            f.argtypes = [_i64p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
-            f.restype = ctypes.c_int
-        L.oracle_vanilla_batch.argtypes = [_i64p, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int, _dp]
-        L.oracle_vanilla_batch.restype = ctypes.c_int
-        L.oracle_vjp_batch.argtypes = [_i64p, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
-        L.oracle_vjp_batch.restype = ctypes.c_int
-        L.oracle_uneven.argtypes = [ctypes.c_int64] + [_dp] * 13
-        L.oracle_uneven.restype = ctypes.c_int
-        L.oracle_max_threads.argtypes = []
-        L.oracle_max_threads.restype = ctypes.c_int
-        _lib = L
+        _lib = _load(_LIB_PATH)
     return _lib
+
+
+_native = None
+
+
+def host_tuned_lib():
+    """The oracle compiled with -march=native for this host, built afresh into
+    a temporary directory (a copy built elsewhere could use instructions this
+    host lacks).  Same results as ``lib()``: no contraction, no fast math."""
+    global _native
+    if _native is None:
+        import tempfile
+
+        out = os.path.join(tempfile.mkdtemp(prefix="mm_oracle_"), "liboracle_native.so")
+        subprocess.run(["make", "-s", "-C", _HERE, "native", f"NATIVE_OUT={out}"], check=True)
+        _native = _load(out)
+    return _native
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    for name in ("oracle_vanilla", "oracle_stable"):
+        f = getattr(L, name)
+        f.argtypes = [_i64p, ctypes.c_int, _dp, _dp, _dp, _dp]
+        f.restype = ctypes.c_int
+    for name in ("oracle_vjp", "oracle_vjp_kahan", "oracle_vjp_kahan_par"):
+        f = getattr(L, name)
+        f.argtypes = [_i64p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        f.restype = ctypes.c_int
+    L.oracle_vanilla_batch.argtypes = [_i64p, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int, _dp]
+    L.oracle_vanilla_batch.restype = ctypes.c_int
+    L.oracle_vjp_batch.argtypes = [_i64p, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.oracle_vjp_batch.restype = ctypes.c_int
+    L.oracle_uneven.argtypes = [ctypes.c_int64] + [_dp] * 13
+    L.oracle_uneven.restype = ctypes.c_int
+    L.oracle_max_threads.argtypes = []
+    L.oracle_max_threads.restype = ctypes.c_int
+    return L
 
 
 def _c128(x) -> np.ndarray:
@@ -144,9 +165,10 @@ def vanilla_batch_vjp(G, c, dLdG):
     return dA, db, dc
 
 
-def uneven(triples, shapes, dLdG=None):
+def uneven(triples, shapes, dLdG=None, host_tuned: bool = False):
     """Per-triple forward (and VJP when ``dLdG`` is given) over triples of
-    uneven mode count and shape, OpenMP across triples.
+    uneven mode count and shape, OpenMP across triples (``host_tuned``: the
+    -march=native build).
 
     Returns ``(Gs, grads)`` with ``Gs`` a list of arrays and ``grads`` a list of
     ``(dA, db, dc)`` or ``None``.
@@ -168,7 +190,7 @@ def uneven(triples, shapes, dLdG=None):
     dA = np.zeros_like(A)
     db = np.zeros_like(b)
     dc = np.zeros_like(c)
-    rc = lib().oracle_uneven(
+    rc = (host_tuned_lib() if host_tuned else lib()).oracle_uneven(
         n, _ptr(dims), _ptr(shp), _ptr(a_off), _ptr(b_off), _ptr(g_off), _ptr(A), _ptr(b), _ptr(c), _ptr(G),
         None if g is None else _ptr(g), _ptr(dA), _ptr(db), _ptr(dc),
     )
