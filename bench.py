@@ -287,8 +287,14 @@ def main():
         log("uneven 4096-ket work-queue batch")
         from mrmustard_amd.autotune import tuned_plan
 
+        # another draw of the same distribution first: the measured batch then
+        # finds every one of its classes in the tuning table
+        _, shapes_other = W.uneven_kets(seed=102)
+        other, other_info = tuned_plan(shapes_other, fused=False)
+        other.close()
         triples, shapes = W.uneven_kets()
         plan, tune_info = tuned_plan(shapes, fused=False)
+        tune_info["other_draw"] = {k: other_info[k] for k in ("tune", "cached", "classes", "classes_timed")}
         Au, bu, cu = plan.pack(triples)
         Gu = torch.empty(plan.n_amplitudes, dtype=torch.complex128, device=dev)
         t_unev = timed(lambda: plan.run(Au, bu, cu, G=Gu, graph=True), max(5, args.steps // 10), 2)
