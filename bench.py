@@ -384,7 +384,45 @@ def main():
             "ket_side_tflops": gflop / t_ket / 1e3,
             "note": "8e10 flop per product; the ket side adds a permutation of rho and of the result (2 x 3.2 GB)",
         }
-        del G4, U2
+        del G4
+        # the conversion fused with the same kind of contraction: wires 6, 7 of
+        # the density matrix contracted with U2 as each slab of the lattice is
+        # made, against the whole lattice followed by contract
+        log("4-mode DM (10,)^8: conversion fused with a two-mode operator")
+        from mrmustard_amd.lattice.fused import fock_contract
+
+        labs, lab2, labo = list(range(8)), [6, 7, "A", "B"], [0, 1, 2, 3, 4, 5, "A", "B"]
+
+        def unfused_op():
+            return contract(st.vanilla_hip(shape4, A4t, b4t, c4t), U2, labs, lab2, labo)
+
+        def fused_op():
+            return fock_contract(A4t, b4t, c4t, shape4, labs, U2, lab2, labo)
+
+        def peak_bytes(fn):
+            torch.cuda.synchronize()
+            base = torch.cuda.memory_allocated()
+            torch.cuda.reset_peak_memory_stats()
+            r = fn()
+            torch.cuda.synchronize()
+            return r, torch.cuda.max_memory_allocated() - base
+
+        r_un, m_un = peak_bytes(unfused_op)
+        r_fu, m_fu = peak_bytes(fused_op)
+        fc_err = float((r_fu - r_un).abs().max() / r_un.abs().max())
+        del r_un, r_fu
+        t_un = timed(unfused_op, 3, 1)
+        t_fu = timed(fused_op, 3, 1)
+        extra["dm8_10x8"]["fused_conversion_contraction"] = {
+            "op": "fock_contract: lattice of the (10,)^8 DM, wires 6, 7 contracted with a (10,10,10,10) operator",
+            "fused_ms": t_fu * 1e3,
+            "unfused_ms": t_un * 1e3,
+            "fused_peak_bytes": int(m_fu),
+            "unfused_peak_bytes": int(m_un),
+            "rel_err_fused_vs_unfused": fc_err,
+            "note": "peak = device memory allocated during the call above what was allocated before it (output included)",
+        }
+        del U2
         # ---- binomial strategy (fill by weight, L2 stop) on one (30,30,30) ket
         log("binomial: one (30,30,30) ket, max_l2 = 0.99999")
         A0, b0, c0 = A[0], b[0], c[0]

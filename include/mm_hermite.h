@@ -237,6 +237,24 @@ size_t mm_hermite_fwd_contract_workspace(const int64_t *shape, int D);
 int mm_hermite_fwd_contract(const double *A, const double *b, const double *c, const int64_t *shape, int D, int n_head,
                             const double *w, double *out, void *workspace, size_t ws_bytes, void *stream);
 
+/* The same for B triples sharing `shape`, contracted with a K x M matrix:
+ * out[t] (prod(shape[:n_head]) x M, row-major) = G_t (rows x K) . W_t with
+ * W_t = W + t * sW complex elements (K x M row-major; sW = 0 shares one
+ * matrix).  Covers a Bargmann state converted to Fock and at once contracted
+ * with a Fock array on its trailing wires -- CircuitComponent.contract's
+ * to_fock of the Bargmann side followed by ArrayAnsatz.contract
+ * (mrmustard/lab/circuit_components.py:445-458,
+ * mrmustard/physics/ansatz/array_ansatz.py:159-225, mm_einsum.py:272-292) --
+ * and the batched derived variables of fock_array (circuit_components.py:
+ * 516-528, M = 1, one w per triple).  Each level-0 slab is contracted as it
+ * is made (M = 1: one dot product per row; M > 1: the matrix cores, as
+ * mm_zgemm), so the workspace is that of one mm_hermite_fwd_contract call for
+ * any B.  A, b, c are packed per triple (D*D, D, 1 complex entries). */
+size_t mm_hermite_batch_fwd_contract_workspace(const int64_t *shape, int D);
+int mm_hermite_batch_fwd_contract(const double *A, const double *b, const double *c, int64_t B, const int64_t *shape, int D,
+                                  int n_head, const double *W, int64_t M, int64_t sW, double *out, void *workspace,
+                                  size_t ws_bytes, void *stream);
+
 /* ------------------------------------------------------ contraction (MFMA)
  *
  * C = A * B for complex128 row-major matrices (A: M x K with row stride lda,

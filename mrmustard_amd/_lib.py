@@ -59,6 +59,8 @@ SIGNATURES = {
     "mm_hermite_1leftover": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _int, _vp, _vp, _sz, _vp]),
     "mm_hermite_fwd_contract_workspace": (_sz, [_vp, _int]),
     "mm_hermite_fwd_contract": (_int, [_vp, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp, _sz, _vp]),
+    "mm_hermite_batch_fwd_contract_workspace": (_sz, [_vp, _int]),
+    "mm_hermite_batch_fwd_contract": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _int, _vp, _i64, _i64, _vp, _vp, _sz, _vp]),
     "mm_hermite_1leftover_vjp_workspace": (_sz, [_i64, _vp, _int]),
     "mm_hermite_1leftover_vjp": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mm_plan_create": (_int, [_i64, _vp, _vp, _int, _vp, _vp]),

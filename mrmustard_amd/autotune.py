@@ -5,7 +5,7 @@ largest axis-0 slab handed to a single wave instead of a workgroup.  The best
 choice depends on the mix of triples, so the tuner times each *class* of
 triple once -- the number of indices D, the size class ceil(log2) of the slab
 prod(shape[1:]), forward or fused -- on a synthetic batch of that class under
-every team option (one wave per triple; one workgroup of 256, 512 or 1024
+every team option (one wave per triple; one workgroup of 256, 512 or 768
 threads), and keeps the cost per triple in a table.  A batch's configuration
 is then read off the table: the threads and the wave threshold that minimise
 the sum of the per-triple costs of its classes.  Only classes missing from
@@ -57,9 +57,10 @@ def class_key(D: int, sc: int, fused: bool, arch: str | None = None) -> str:
 
 
 def team_threads(fused: bool, max_d: int) -> list[int]:
+    """Workgroup sizes a plan accepts (the LDS kernel's launch bound)."""
     if fused:
         return [256] if max_d > 3 else [256, 512]
-    return [256, 512] if max_d > 3 else [256, 512, 1024]
+    return [256, 512] if max_d > 3 else [256, 512, 768]
 
 
 def _load_table() -> dict:

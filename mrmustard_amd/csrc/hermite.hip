@@ -1381,11 +1381,21 @@ size_t mm_hermite_fwd_contract_workspace(const int64_t *shape, int D) {
 
 int mm_hermite_fwd_contract(const double *A, const double *b, const double *c, const int64_t *shape, int D, int n_head,
                             const double *w, double *out, void *workspace, size_t ws_bytes, void *stream) {
+    return mm_hermite_batch_fwd_contract(A, b, c, 1, shape, D, n_head, w, 1, 0, out, workspace, ws_bytes, stream);
+}
+
+size_t mm_hermite_batch_fwd_contract_workspace(const int64_t *shape, int D) { return mm_hermite_fwd_contract_workspace(shape, D); }
+
+int mm_hermite_batch_fwd_contract(const double *A, const double *b, const double *c, int64_t B, const int64_t *shape, int D, int n_head,
+                                  const double *W, int64_t M, int64_t sW, double *out, void *workspace, size_t ws_bytes, void *stream) {
     Shape S;
     int rc = make_shape(shape, D, &S);
     if (rc) return rc;
     if (n_head < 1 || n_head > D) return fail(MM_ERR_INVALID, "n_head=%d outside [1, %d]", n_head, D);
-    if (!A || !b || !c || !w || !out) return fail(MM_ERR_INVALID, "NULL array pointer");
+    if (B < 0 || M < 1 || sW < 0) return fail(MM_ERR_INVALID, "B=%lld, M=%lld, sW=%lld: need B >= 0, M >= 1, sW >= 0", (long long)B,
+                                              (long long)M, (long long)sW);
+    if (B == 0) return MM_OK;
+    if (!A || !b || !c || !W || !out) return fail(MM_ERR_INVALID, "NULL array pointer");
     DevState *ds = nullptr;
     rc = device_state(&ds);
     if (rc) return rc;
@@ -1394,30 +1404,41 @@ int mm_hermite_fwd_contract(const double *A, const double *b, const double *c, c
         return fail(MM_ERR_WORKSPACE, "fused contraction workspace of %zu bytes needed, %zu given", need, ws_bytes);
     int64_t K = 1;
     for (int j = n_head; j < D; ++j) K *= shape[j];
+    const int64_t rows = S.N / K;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const double2 *w2 = reinterpret_cast<const double2 *>(w);
-    double2 *o2 = reinterpret_cast<double2 *>(out);
-    if (!contract_windowed(S, ds)) {
-        double2 *G = reinterpret_cast<double2 *>(workspace);
-        rc = vanilla_uniform(S, 1, A, b, c, nullptr, reinterpret_cast<double *>(G), nullptr, nullptr, nullptr, nullptr, 0, ds, s);
-        if (rc) return rc;
-        return launch_rows_dot(G, w2, o2, S.N / K, K, s);
-    }
-    // rows of one level-0 slab: n_head >= 1, so K divides stride[0]
-    const int64_t S0 = S.st[0], R = S0 / K;
+    // rows x K amplitudes times the K x M matrix of triple t into rows x M outputs
+    auto contract_rows = [&](const double2 *Gr, int64_t nrows, const double *Wt, double *ot) -> int {
+        if (M == 1) return launch_rows_dot(Gr, reinterpret_cast<const double2 *>(Wt), reinterpret_cast<double2 *>(ot), nrows, K, s);
+        return mm_zgemm(reinterpret_cast<const double *>(Gr), K, Wt, M, ot, M, nrows, M, K, s);
+    };
+    const bool windowed = contract_windowed(S, ds);
+    const int64_t S0 = S.st[0], R = S0 / K;  // rows of one level-0 slab: n_head >= 1, so K divides stride[0]
     const size_t slab_bytes = align256(size_t(S0) * 16);
     char *base = reinterpret_cast<char *>(workspace);
     double2 *slot[3] = {reinterpret_cast<double2 *>(base), reinterpret_cast<double2 *>(base + slab_bytes),
                         reinterpret_cast<double2 *>(base + 2 * slab_bytes)};
-    rc = window_face(S, A, b, c, slot[0], ds, s);
-    if (rc) return rc;
-    for (int m = 0; m < S.s[0]; ++m) {
-        if (m >= 1) {
-            rc = window_slab0(S, A, b, slot[m % 3], slot[(m - 1) % 3], m >= 2 ? slot[(m - 2) % 3] : slot[(m - 1) % 3], m, ds, s);
+    for (int64_t t = 0; t < B; ++t) {
+        const double *At = A + 2 * t * S.D * S.D, *bt = b + 2 * t * S.D, *ct = c + 2 * t;
+        const double *Wt = W + 2 * t * sW;
+        double *ot = out + 2 * t * rows * M;
+        if (!windowed) {
+            double2 *G = reinterpret_cast<double2 *>(workspace);
+            rc = vanilla_uniform(S, 1, At, bt, ct, nullptr, reinterpret_cast<double *>(G), nullptr, nullptr, nullptr, nullptr, 0, ds, s);
+            if (rc) return rc;
+            rc = contract_rows(G, rows, Wt, ot);
+            if (rc) return rc;
+            continue;
+        }
+        rc = window_face(S, At, bt, ct, slot[0], ds, s);
+        if (rc) return rc;
+        for (int m = 0; m < S.s[0]; ++m) {
+            if (m >= 1) {
+                rc = window_slab0(S, At, bt, slot[m % 3], slot[(m - 1) % 3], m >= 2 ? slot[(m - 2) % 3] : slot[(m - 1) % 3], m, ds, s);
+                if (rc) return rc;
+            }
+            rc = contract_rows(slot[m % 3], R, Wt, ot + 2 * int64_t(m) * R * M);
             if (rc) return rc;
         }
-        rc = launch_rows_dot(slot[m % 3], w2, o2 + int64_t(m) * R, R, K, s);
-        if (rc) return rc;
     }
     return MM_OK;
 }

@@ -14,10 +14,10 @@ n + m variables (batch first) and c of shape ``batch + shape_derived_vars``.
 Both run the batched Hermite kernels and one batched product on the FP64
 matrix cores (``zgemm_batched_hip``) for the contraction with c.  For large
 lattices ``fock_array_derived`` fuses the conversion with the contraction
-(``mm_hermite_fwd_contract``): the lattice is computed one slab of its first
-axis at a time in a three-slab window and each slab is contracted with c as
-soon as it exists, so the lattice on ``shape + shape_derived_vars`` is never
-stored whole.
+(``mm_hermite_batch_fwd_contract``, every triple of the batch in one call):
+the lattice is computed one slab of its first axis at a time in a three-slab
+window and each slab is contracted with c as soon as it exists, so the
+lattice on ``shape + shape_derived_vars`` is never stored whole.
 """
 
 from __future__ import annotations
@@ -65,21 +65,24 @@ def _contract_derived(G, c, batch_shape, head_shape):
     return zgemm_batched_hip(G3, c3).reshape(tuple(batch_shape) + tuple(head_shape))
 
 
-def _fused_one(A1, b1, c1, full_shape, n):
-    """One triple through mm_hermite_fwd_contract: the lattice of (A1, b1, 1)
-    on ``full_shape`` contracted with c1 over its last axes as it is made."""
+def _fused_batch(A3, b3, c3, full_shape, n):
+    """All triples through one mm_hermite_batch_fwd_contract call: the lattice
+    of (A_t, b_t, 1) on ``full_shape`` contracted with c_t over its last axes
+    as it is made, one slab at a time."""
     L = _lib.load()
     s = _shape_arr(full_shape)
     D = len(full_shape)
-    one = torch.ones((), dtype=C128, device=A1.device)
-    w = c1.reshape(-1).contiguous()
-    out = torch.empty(full_shape[:n], dtype=C128, device=A1.device)
-    ws = _workspace(L.mm_hermite_fwd_contract_workspace(s.ctypes.data, D))
-    rc = L.mm_hermite_fwd_contract(
-        _ptr(A1.contiguous()), _ptr(b1.contiguous()), _ptr(one), s.ctypes.data, D, n, _ptr(w), _ptr(out),
+    nb = A3.shape[0]
+    ones = torch.ones(nb, dtype=C128, device=A3.device)
+    K = pymath.prod(full_shape[n:])
+    w = c3.reshape(nb, K).contiguous()
+    out = torch.empty((nb,) + tuple(full_shape[:n]), dtype=C128, device=A3.device)
+    ws = _workspace(L.mm_hermite_batch_fwd_contract_workspace(s.ctypes.data, D))
+    rc = L.mm_hermite_batch_fwd_contract(
+        _ptr(A3.contiguous()), _ptr(b3.contiguous()), _ptr(ones), nb, s.ctypes.data, D, n, _ptr(w), 1, K, _ptr(out),
         _ptr(ws), 0 if ws is None else ws.numel(), _stream(),
     )
-    _lib.check(rc, "mm_hermite_fwd_contract")
+    _lib.check(rc, "mm_hermite_batch_fwd_contract")
     return out
 
 
@@ -110,8 +113,7 @@ def fock_array_derived(A, b, c, shape, fused: bool | None = None) -> torch.Tenso
         A3 = A.reshape(nb, *A.shape[-2:])
         b3 = b.reshape(nb, b.shape[-1])
         c3 = c.reshape(nb, *shape_derived)
-        outs = [_fused_one(A3[k], b3[k], c3[k], full, n) for k in range(nb)]
-        return torch.stack(outs).reshape(tuple(batch_shape) + shape)
+        return _fused_batch(A3, b3, c3, full, n).reshape(tuple(batch_shape) + shape)
     G = _lattice(A, b, batch_shape, full)
     return _contract_derived(G, c, batch_shape, shape)
 
