@@ -278,14 +278,113 @@ def _take_diagonals(t, labs):
     return t, labs
 
 
+def greedy_path(labels, output, sizes):
+    """Pairwise contraction order of opt_einsum's "greedy" optimizer
+    (opt_einsum >= 3.4, ``paths.greedy`` with its default memory-removed
+    cost: the path ``math.einsum`` asks for, backend_manager.py:526-548),
+    restated on label sets:
+      1. operands with equal label sets are multiplied first, left to right;
+      2. every pair of operands sharing a label that the output lacks is a
+         candidate, scored when it is found by (size of its result - sizes
+         of the two operands, later operand id, earlier operand id); the best
+         candidate whose operands are both still alive is contracted, and the
+         result's pairs are added (scores are not revised afterwards, as in
+         opt_einsum's lazy heap);
+      3. what is left is multiplied as outer products, smallest first.
+    A result keeps the labels of the output and those another operand still
+    carries.  Returns steps (i, j, keep) over SSA ids: inputs are 0..n-1 and
+    every product gets the next id.
+    """
+    import heapq
+
+    output = frozenset(output)
+    n = len(labels)
+    if n <= 1:
+        return []
+    first = {lab: k for k, lab in enumerate(dict.fromkeys(x for lb in labels for x in lb))}
+
+    def ordered(k):
+        return sorted(k, key=first.__getitem__)
+
+    def size(k):
+        m = 1
+        for lab in k:
+            m *= sizes[lab]
+        return m
+
+    steps, nxt = [], n
+    alive = {}  # label set -> id of the operand holding it
+    for sid, lb in enumerate(labels):
+        k = frozenset(lb)
+        if k in alive:
+            steps.append((alive[k], sid, ordered(k)))
+            alive[k] = nxt
+            nxt += 1
+        else:
+            alive[k] = sid
+    holders = {}  # label outside the output -> label sets holding it
+    for k in alive:
+        for lab in k - output:
+            holders.setdefault(lab, set()).add(k)
+
+    def count(lab):
+        return len(holders.get(lab, ()))
+
+    def candidate(k1, k2):
+        both, either = k1 & k2, k1 | k2
+        k12 = frozenset(lab for lab in either if lab in output or count(lab) >= (3 if lab in both else 2))
+        i1, i2 = alive[k1], alive[k2]
+        if i1 > i2:
+            k1, k2, i1, i2 = k2, k1, i2, i1
+        return (size(k12) - size(k1) - size(k2), i2, i1), k1, k2, k12
+
+    queue = []
+    for lab, ks in holders.items():
+        ks = sorted(ks, key=alive.__getitem__)
+        for a in range(len(ks) - 1):
+            for kb in ks[a + 1 :]:
+                heapq.heappush(queue, candidate(ks[a], kb))
+    while queue:
+        _, k1, k2, k12 = heapq.heappop(queue)
+        if k1 not in alive or k2 not in alive:
+            continue  # an operand of this candidate is gone
+        i1, i2 = alive.pop(k1), alive.pop(k2)
+        for lab in k1 - output:
+            holders[lab].discard(k1)
+        for lab in k2 - output:
+            holders[lab].discard(k2)
+        steps.append((i1, i2, ordered(k12)))
+        if k12 in alive:
+            steps.append((alive[k12], nxt, ordered(k12)))
+            nxt += 1
+        else:
+            for lab in k12 - output:
+                holders[lab].add(k12)
+        alive[k12] = nxt
+        nxt += 1
+        partners = {k for lab in k12 - output for k in holders[lab]} - {k12}
+        for k in sorted(partners, key=alive.__getitem__):
+            heapq.heappush(queue, candidate(k12, k))
+    heap = [(size(k & output), sid, k) for k, sid in alive.items()]
+    heapq.heapify(heap)
+    _, s1, k1 = heapq.heappop(heap)
+    while heap:
+        _, s2, k2 = heapq.heappop(heap)
+        k12 = (k1 | k2) & output
+        steps.append((min(s1, s2), max(s1, s2), ordered(k12)))
+        _, s1, k1 = heapq.heappushpop(heap, (size(k12), nxt, k12))
+        nxt += 1
+    return steps
+
+
 def einsum(string: str, *tensors) -> torch.Tensor:
     """``math.einsum`` (backend_manager.py:526-548) on the GPU, with numpy's
     string rules (``...``, implicit output, a label repeated within an operand
     for its diagonal, size-1 axes broadcast).  Operands are contracted
     pairwise, each pair as one batched product on the matrix cores
-    (``contract``), in a greedy order: the pair whose result is smallest first,
-    as opt_einsum's "greedy" prefers; an intermediate keeps the labels the
-    output or another operand still needs.  Axes of one label must have equal
+    (``contract``), in the order of opt_einsum's "greedy" path
+    (``greedy_path``); an intermediate keeps the labels the output or another
+    operand still needs.  Axes of one label must have equal
     sizes or size 1 (ValueError otherwise, as numpy)."""
     if not tensors:
         raise ValueError("einsum needs at least one operand")
@@ -314,24 +413,15 @@ def einsum(string: str, *tensors) -> torch.Tensor:
     # size-1 axes broadcast against the others, as numpy's do
     pairs = [(t.expand(*[sizes[lab] for lab in labs]) if any(int(n) != sizes[lab] for lab, n in zip(labs, t.shape)) else t, labs)
              for t, labs in pairs]
-    # greedy order: contract the pair whose result is smallest (ties: leftmost)
-    while len(pairs) > 1:
-        best = None
-        for i in range(len(pairs)):
-            for j in range(i + 1, len(pairs)):
-                rest = [lb for k, (_, lb) in enumerate(pairs) if k not in (i, j)]
-                later = set(outl).union(*[set(lb) for lb in rest])
-                keep = list(dict.fromkeys(lab for lab in pairs[i][1] + pairs[j][1] if lab in later))
-                n_out = 1
-                for lab in keep:
-                    n_out *= sizes[lab]
-                if best is None or n_out < best[0]:
-                    best = (n_out, i, j, keep)
-        _, i, j, keep = best
-        (ti, li), (tj, lj) = pairs[i], pairs[j]
-        merged = (contract(ti, tj, li, lj, keep), keep)
-        pairs = [p for k, p in enumerate(pairs) if k not in (i, j)]
-        pairs.insert(i, merged)
+    # the contraction order of opt_einsum's "greedy" path (the reference's
+    # default, settings.EINSUM_OPTIMIZE), each pair one batched product
+    ops = dict(enumerate(pairs))
+    nxt = len(pairs)
+    for i, j, keep in greedy_path([lb for _, lb in pairs], outl, sizes):
+        (ti, li), (tj, lj) = ops.pop(i), ops.pop(j)
+        ops[nxt] = (contract(ti, tj, li, lj, keep), keep)
+        nxt += 1
+    pairs = list(ops.values())
     res, rl = pairs[0]
     # one operand left: sum what the output does not name, then order the axes
     lone = [i for i, lab in enumerate(rl) if lab not in outl]

@@ -177,9 +177,10 @@ class BackendManager:
         r"""Einstein summation of Fock-basis arrays on the FP64 matrix cores
         (backend_manager.py:526-548), as ``ArrayAnsatz.contract`` and the
         derived-variables branch of ``fock_array`` call it: numpy's string
-        rules, operands contracted pairwise in a greedy order (the pair with
-        the smallest result first).  ``optimize`` and ``backend`` are accepted
-        for the reference's signature and ignored."""
+        rules, operands contracted pairwise in the order of opt_einsum's
+        greedy path (the reference's default ``optimize="greedy"``).  Other
+        ``optimize`` values and ``backend`` are accepted for the reference's
+        signature and take the same order."""
         from ..contraction import einsum as _einsum
 
         return _einsum(string, *tensors)

@@ -260,3 +260,32 @@ def test_einsum_three_operands_gradient_and_errors():
         assert np.max(np.abs(x.grad.cpu().numpy() - r.grad.numpy())) < 1e-12 * np.max(np.abs(r.grad.numpy()))
     with pytest.raises(ValueError, match="sizes"):
         einsum("ij,jk->ik", torch.zeros((2, 3), device="cuda"), torch.zeros((4, 2), device="cuda"))
+
+
+def test_greedy_path_hand_derived():
+    """opt_einsum's greedy order (restated in greedy_path; opt_einsum itself is
+    not installed), on cases worked out by hand from its rules: equal label
+    sets first, then the candidate removing the most memory (ties to the
+    earliest operands), results keeping only labels still needed, outer
+    products of the smallest last."""
+    from mrmustard_amd.contraction import greedy_path
+
+    # chain a-b-c-d, sizes a=2 b=100 c=2 d=100: (ab,bc) removes 200+200-4,
+    # (bc,cd) would grow to 10^4; then the result ac meets cd
+    sz = {"a": 2, "b": 100, "c": 2, "d": 100}
+    assert greedy_path([["a", "b"], ["b", "c"], ["c", "d"]], ["a", "d"], sz) == [(0, 1, ["a", "c"]), (2, 3, ["a", "d"])]
+    # the same chain given in another order: the cheap pair (ab, bc) is still first
+    steps = greedy_path([["a", "b"], ["c", "d"], ["b", "c"]], ["a", "d"], sz)
+    assert {steps[0][0], steps[0][1]} == {0, 2} and steps[0][2] == ["a", "c"]
+    # equal costs: the pair with the earliest later operand wins
+    eq = {x: 5 for x in "abcd"}
+    assert greedy_path([["a", "b"], ["b", "c"], ["c", "d"]], ["a", "d"], eq)[0] == (0, 1, ["a", "c"])
+    # equal label sets are multiplied first, then summed against the rest
+    assert greedy_path([["i", "j"], ["k"], ["i", "j"]], ["k"], {"i": 3, "j": 4, "k": 5}) == [(0, 2, ["i", "j"]), (1, 3, ["k"])]
+    # a label held by three operands stays in the first pair's result and
+    # is summed by the second
+    three = greedy_path([["x", "a"], ["x", "b"], ["x", "c"]], ["a", "b", "c"], {"x": 4, "a": 2, "b": 2, "c": 2})
+    assert three == [(0, 1, ["x", "a", "b"]), (2, 3, ["a", "b", "c"])]
+    # no shared label: outer products, smallest first
+    assert greedy_path([["a"], ["b"], ["c"]], ["a", "b", "c"], {"a": 9, "b": 2, "c": 3}) == [(1, 2, ["b", "c"]), (0, 3, ["a", "b", "c"])]
+    assert greedy_path([["a", "b"]], ["a"], {"a": 2, "b": 2}) == []
