@@ -382,7 +382,7 @@ def main():
             "bra_side_tflops": gflop / t_bra / 1e3,
             "ket_side_ms": t_ket * 1e3,
             "ket_side_tflops": gflop / t_ket / 1e3,
-            "note": "8e10 flop per product; the ket side adds a permutation of rho and of the result (2 x 3.2 GB)",
+            "note": "8e10 flop per product; the ket side reads rho in place as 10^4 products (100 x 100 x 100) over its bra indices",
         }
         del G4
         # the conversion fused with the same kind of contraction: wires 6, 7 of

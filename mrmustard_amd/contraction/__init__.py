@@ -149,6 +149,44 @@ def _zgemm_batched(A, B) -> torch.Tensor:
     return out
 
 
+def _middle_block(x, lx, y, ly, idx_out):
+    """x[lead, s, trail] contracted with y[s, free] when the output is
+    (lead, free, trail): one batched product per lead index, C = y^T x_lead,
+    with x read in place (batch = prod(lead), K = prod(s), N = prod(trail))
+    and only y permuted.  None when the labels do not have that form."""
+    shared = set(lx) & set(ly)
+    if not shared or any(lab in idx_out for lab in shared):
+        return None
+    pos = [k for k, lab in enumerate(lx) if lab in shared]
+    if pos != list(range(pos[0], pos[0] + len(pos))):
+        return None
+    summed = [lx[k] for k in pos]
+    lead, trail = lx[: pos[0]], lx[pos[-1] + 1 :]
+    if not lead and not trail:
+        return None
+    nl, nt = len(lead), len(trail)
+    if list(idx_out[:nl]) != lead or list(idx_out[len(idx_out) - nt :]) != trail:
+        return None
+    mid = list(idx_out[nl : len(idx_out) - nt])
+    free = [lab for lab in ly if lab not in shared]
+    if sorted(map(str, mid)) != sorted(map(str, free)) or len(mid) != len(free):
+        return None
+
+    def prod(t, labs, lab_list):
+        n = 1
+        for lab in labs:
+            n *= int(t.shape[lab_list.index(lab)])
+        return n
+
+    nb, K, N = prod(x, lead, lx), prod(x, summed, lx), prod(x, trail, lx)
+    M = prod(y, mid, ly)
+    yt = y.permute(tuple(ly.index(lab) for lab in mid + summed)).reshape(1, M, K).contiguous()
+    C = zgemm_batched_hip(yt, x.contiguous().reshape(nb, K, N))
+    dims = [int(x.shape[lx.index(lab)]) for lab in lead] + [int(y.shape[ly.index(lab)]) for lab in mid] + \
+        [int(x.shape[lx.index(lab)]) for lab in trail]
+    return C.reshape(dims)
+
+
 def contract(array1, array2, idx1, idx2, idx_out) -> torch.Tensor:
     """``ArrayAnsatz.contract`` (array_ansatz.py:159-225) of two arrays on the
     GPU: labels (str or int) name the axes of each array; shared labels whose
@@ -202,6 +240,12 @@ def contract(array1, array2, idx1, idx2, idx_out) -> torch.Tensor:
                 a = t
             else:
                 b = t
+    # an operand whose summed axes form one block between axes the output
+    # keeps in place is read where it lies (no permutation of it or of the result)
+    for x, lx, y, ly in sorted(((a, idx1, b, idx2), (b, idx2, a, idx1)), key=lambda t: -t[0].numel()):
+        r = _middle_block(x, lx, y, ly, idx_out)
+        if r is not None:
+            return r
     batch = [lab for lab in idx_out if lab in shared]
     summed = [lab for lab in idx1 if lab in shared and lab not in idx_out]
     rows = [lab for lab in idx_out if lab in idx1 and lab not in shared]

@@ -289,3 +289,29 @@ def test_greedy_path_hand_derived():
     # no shared label: outer products, smallest first
     assert greedy_path([["a"], ["b"], ["c"]], ["a", "b", "c"], {"a": 9, "b": 2, "c": 3}) == [(1, 2, ["b", "c"]), (0, 3, ["a", "b", "c"])]
     assert greedy_path([["a", "b"]], ["a"], {"a": 2, "b": 2}) == []
+
+
+def test_contract_middle_block_on_cpu(monkeypatch):
+    """A wire block in the middle of the larger operand, with the output
+    keeping its other axes in place (the ket side of an operator on a density
+    matrix): read in place, same result as numpy."""
+    C = _einsum_on_cpu(monkeypatch)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 3, 4, 5)) + 1j * rng.standard_normal((2, 3, 4, 5))
+    y = rng.standard_normal((4, 3, 6)) + 1j * rng.standard_normal((4, 3, 6))
+    got = C.contract(x, y, [0, 1, 2, 3], [2, 1, "y"], [0, "y", 3]).numpy()
+    assert np.allclose(got, np.einsum("abcd,cby->ayd", x, y), rtol=1e-13, atol=1e-13)
+    assert C._middle_block(torch_t(x), [0, 1, 2, 3], torch_t(y), [2, 1, "y"], [0, "y", 3]) is not None
+    # the output in another order is not that form: the generic path
+    got2 = C.contract(x, y, [0, 1, 2, 3], [2, 1, "y"], [3, 0, "y"]).numpy()
+    assert np.allclose(got2, np.einsum("abcd,cby->day", x, y), rtol=1e-13, atol=1e-13)
+    assert C._middle_block(torch_t(x), [0, 1, 2, 3], torch_t(y), [2, 1, "y"], [3, 0, "y"]) is None
+    # summed axes not adjacent in x: not that form
+    z = rng.standard_normal((3, 5, 2)) + 1j * rng.standard_normal((3, 5, 2))
+    assert C._middle_block(torch_t(x), [0, 1, 2, 3], torch_t(z), [1, 3, "z"], [0, "z", 2]) is None
+
+
+def torch_t(a):
+    import torch
+
+    return torch.as_tensor(a)
