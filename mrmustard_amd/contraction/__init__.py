@@ -153,7 +153,9 @@ def _middle_block(x, lx, y, ly, idx_out):
     """x[lead, s, trail] contracted with y[s, free] when the output is
     (lead, free, trail): one batched product per lead index, C = y^T x_lead,
     with x read in place (batch = prod(lead), K = prod(s), N = prod(trail))
-    and only y permuted.  None when the labels do not have that form."""
+    and only y permuted.  None when the labels do not have that form, when
+    trail is empty (the generic product then reads x in place too) or when
+    the products would be narrow (N < 32 with more than one)."""
     shared = set(lx) & set(ly)
     if not shared or any(lab in idx_out for lab in shared):
         return None
@@ -162,7 +164,7 @@ def _middle_block(x, lx, y, ly, idx_out):
         return None
     summed = [lx[k] for k in pos]
     lead, trail = lx[: pos[0]], lx[pos[-1] + 1 :]
-    if not lead and not trail:
+    if not trail:  # x is then already (rows, summed): the generic product reads it in place
         return None
     nl, nt = len(lead), len(trail)
     if list(idx_out[:nl]) != lead or list(idx_out[len(idx_out) - nt :]) != trail:
@@ -180,6 +182,8 @@ def _middle_block(x, lx, y, ly, idx_out):
 
     nb, K, N = prod(x, lead, lx), prod(x, summed, lx), prod(x, trail, lx)
     M = prod(y, mid, ly)
+    if N < 32 and nb > 1:  # narrow products: one permutation and a single wide product do better
+        return None
     yt = y.permute(tuple(ly.index(lab) for lab in mid + summed)).reshape(1, M, K).contiguous()
     C = zgemm_batched_hip(yt, x.contiguous().reshape(nb, K, N))
     dims = [int(x.shape[lx.index(lab)]) for lab in lead] + [int(y.shape[ly.index(lab)]) for lab in mid] + \

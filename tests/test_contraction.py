@@ -297,7 +297,7 @@ def test_contract_middle_block_on_cpu(monkeypatch):
     matrix): read in place, same result as numpy."""
     C = _einsum_on_cpu(monkeypatch)
     rng = np.random.default_rng(5)
-    x = rng.standard_normal((2, 3, 4, 5)) + 1j * rng.standard_normal((2, 3, 4, 5))
+    x = rng.standard_normal((2, 3, 4, 40)) + 1j * rng.standard_normal((2, 3, 4, 40))
     y = rng.standard_normal((4, 3, 6)) + 1j * rng.standard_normal((4, 3, 6))
     got = C.contract(x, y, [0, 1, 2, 3], [2, 1, "y"], [0, "y", 3]).numpy()
     assert np.allclose(got, np.einsum("abcd,cby->ayd", x, y), rtol=1e-13, atol=1e-13)
@@ -307,7 +307,10 @@ def test_contract_middle_block_on_cpu(monkeypatch):
     assert np.allclose(got2, np.einsum("abcd,cby->day", x, y), rtol=1e-13, atol=1e-13)
     assert C._middle_block(torch_t(x), [0, 1, 2, 3], torch_t(y), [2, 1, "y"], [3, 0, "y"]) is None
     # summed axes not adjacent in x: not that form
-    z = rng.standard_normal((3, 5, 2)) + 1j * rng.standard_normal((3, 5, 2))
+    z = rng.standard_normal((3, 40, 2)) + 1j * rng.standard_normal((3, 40, 2))
+    # trailing summed wires: the generic product, which reads x in place
+    w = rng.standard_normal((4, 40, 7)) + 1j * rng.standard_normal((4, 40, 7))
+    assert C._middle_block(torch_t(x), [0, 1, 2, 3], torch_t(w), [2, 3, "q"], [0, 1, "q"]) is None
     assert C._middle_block(torch_t(x), [0, 1, 2, 3], torch_t(z), [1, 3, "z"], [0, "z", 2]) is None
 
 
