@@ -104,6 +104,48 @@ __device__ __forceinline__ double2 cdiv_py(double2 a, double2 b) {
     return z;
 }
 
+// ------------------------------------------------------------ XCD placement
+// Workgroups of a 1-D grid are dispatched to the 8 XCDs round-robin, so
+// block b runs on XCD b % 8, and each XCD has its own L2.  This maps block b
+// to a logical index such that each XCD holds a contiguous range of them: work
+// split by logical index then keeps neighbouring chunks (which read the same
+// data) on one L2.  Any grid size works; the mapping is a permutation and
+// only placement, not results, depends on it.
+constexpr int MM_XCDS = 8;
+
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n) {
+    const int64_t per = n / MM_XCDS, full = per * MM_XCDS;
+    if (b >= full) return b;  // the remainder keeps its own order
+    return (b % MM_XCDS) * per + b / MM_XCDS;
+}
+
+// The points q0, q0 + inc, ... below hi of a slab of S points that block blk
+// of nblk (256 threads each) visits.  With nblk a multiple of the XCD count,
+// the blocks of one XCD (consecutive logical indices, xcd_block) share one
+// contiguous eighth of the slab and walk it in order, so the points that read
+// the same lines of the slabs below them do so through one L2; otherwise
+// every block strides over the whole slab.
+struct Span {
+    int64_t q0, inc, hi;
+};
+
+__device__ __forceinline__ Span xcd_span(int64_t blk, int64_t nblk, int64_t S) {
+    Span s;
+    if (nblk >= MM_XCDS && nblk % MM_XCDS == 0) {
+        const int64_t per = nblk / MM_XCDS, x = blk / per, j = blk - x * per;
+        const int64_t chunk = ((S + MM_XCDS - 1) / MM_XCDS + 255) / 256 * 256;
+        const int64_t lo = x * chunk;
+        s.hi = lo + chunk < S ? lo + chunk : S;
+        s.q0 = lo + j * 256 + threadIdx.x;
+        s.inc = per * 256;
+    } else {
+        s.q0 = blk * 256 + threadIdx.x;
+        s.inc = nblk * 256;
+        s.hi = S;
+    }
+    return s;
+}
+
 // ------------------------------------------------------------ wave helpers
 
 __device__ __forceinline__ void wave_sync() {

@@ -61,9 +61,9 @@ template <int D>
 __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
     const int d = a.d, m = a.m;
     const int64_t Sd = a.g.stride[d];
-    const int64_t q0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t inc = int64_t(gridDim.x) * blockDim.x;
-    if (q0 >= Sd) return;
+    const Span sp = xcd_span(xcd_block(blockIdx.x, gridDim.x), gridDim.x, Sd);
+    const int64_t q0 = sp.q0, inc = sp.inc, hi = sp.hi;
+    if (q0 >= hi) return;
     int k[D], ki[D];
     digits_of<D>(q0, a.g, d + 1, k);
     digits_of<D>(inc < Sd ? inc : 0, a.g, d + 1, ki);
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
     double2 *G = a.G;
     const int64_t base = int64_t(m) * Sd;
     bool bad = false;
-    for (int64_t q = q0; q < Sd; q += inc) {
+    for (int64_t q = q0; q < hi; q += inc) {
         const int64_t f = base + q;
         double2 v = cmul(bd, G[f - Sd]);
         if (m >= 2) v = cadd(v, cmul(cdd, G[f - 2 * Sd]));
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
         const double2 out = cdiv_by(v, sm, rm);
         G[f] = out;
         bad = bad || !finite2(out);
-        if (q + inc < Sd) digits_add<D>(k, ki, a.g, d + 1);
+        if (q + inc < hi) digits_add<D>(k, ki, a.g, d + 1);
     }
     if (__builtin_expect(bad, 0) && a.nf) atomicOr(a.nf, 1);
 }
@@ -109,9 +109,9 @@ template <int D>
 __device__ __forceinline__ void fwd_slab0_body(const Slab0Args &a, int64_t blk, int64_t nblk) {
     const int m = a.m;
     const int64_t S = a.g.stride[0];
-    const int64_t q0 = blk * 256 + threadIdx.x;
-    const int64_t inc = nblk * 256;
-    if (q0 >= S) return;
+    const Span sp = xcd_span(blk, nblk, S);
+    const int64_t q0 = sp.q0, inc = sp.inc, hi = sp.hi;
+    if (q0 >= hi) return;
     int k[D], ki[D];
     digits_of<D>(q0, a.g, 1, k);
     digits_of<D>(inc < S ? inc : 0, a.g, 1, ki);
@@ -121,20 +121,20 @@ __device__ __forceinline__ void fwd_slab0_body(const Slab0Args &a, int64_t blk, 
     double2 A0j[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) A0j[j] = (j > 0) ? a.A[j] : make_double2(0.0, 0.0);
-    for (int64_t q = q0; q < S; q += inc) {
+    for (int64_t q = q0; q < hi; q += inc) {
         double2 v = cmul(b0, a.p1[q]);
         if (m >= 2) v = cadd(v, cmul(c00, a.p2[q]));
 #pragma unroll
         for (int j = 1; j < D; ++j)
             if (k[j] > 0) v = cadd(v, cmul(cscale_promoted(A0j[j], a.sq[k[j]]), a.p1[q - a.g.stride[j]]));
         a.cur[q] = cdiv_by(v, sm, rm);
-        if (q + inc < S) digits_add<D>(k, ki, a.g, 1);
+        if (q + inc < hi) digits_add<D>(k, ki, a.g, 1);
     }
 }
 
 template <int D>
 __global__ __launch_bounds__(256) void fwd_slab0_kernel(Slab0Args a) {
-    fwd_slab0_body<D>(a, blockIdx.x, gridDim.x);
+    fwd_slab0_body<D>(a, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 __global__ void zero_c128_kernel(double2 *p, int64_t n) {
@@ -275,9 +275,9 @@ __device__ __forceinline__ void vjp_slab_body(const SlabVjpArgs &a, int64_t blk,
     double2 acc[NL];
 #pragma unroll
     for (int e = 0; e < NL; ++e) acc[e] = make_double2(0.0, 0.0);
-    const int64_t q0 = blk * 256 + threadIdx.x;
-    const int64_t inc = nblk * 256;
-    if (q0 < S) {
+    const Span sp = xcd_span(blk, nblk, S);
+    const int64_t q0 = sp.q0, inc = sp.inc, hi = sp.hi;
+    if (q0 < hi) {
         int64_t st[D];
 #pragma unroll
         for (int i = 0; i < D; ++i) st[i] = a.g.stride[i];
@@ -288,7 +288,7 @@ __device__ __forceinline__ void vjp_slab_body(const SlabVjpArgs &a, int64_t blk,
         ki[0] = 0;
         const double2 *p1 = m >= 1 ? a.p1 : a.cur;
         const double2 *p2 = m >= 2 ? a.p2 : a.cur;
-        for (int64_t q = q0; q < S; q += inc) {
+        for (int64_t q = q0; q < hi; q += inc) {
             const double2 gk = a.gg[q];
             if (R0 == 0) {
                 acc[RG::lc()] = cfma(a.cur[q], gk, acc[RG::lc()]);
@@ -323,7 +323,7 @@ __device__ __forceinline__ void vjp_slab_body(const SlabVjpArgs &a, int64_t blk,
                     acc[RG::la(i, j)] = cfma(make_double2(w * t3.x, w * t3.y), ui, acc[RG::la(i, j)]);
                 }
             }
-            if (q + inc < S) digits_add<D>(k, ki, a.g, 1);
+            if (q + inc < hi) digits_add<D>(k, ki, a.g, 1);
         }
     }
     block_store_group<D, R0, R1, true>(acc, a.partial + blk * RG::NA);
@@ -331,7 +331,7 @@ __device__ __forceinline__ void vjp_slab_body(const SlabVjpArgs &a, int64_t blk,
 
 template <int D, int R0, int R1>
 __global__ __launch_bounds__(256) void vjp_slab_kernel(SlabVjpArgs a) {
-    vjp_slab_body<D, R0, R1>(a, blockIdx.x, gridDim.x);
+    vjp_slab_body<D, R0, R1>(a, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // Row groups per D: one for D <= 4, two for D = 5, 6, three for D = 7, 8.
@@ -344,16 +344,18 @@ struct RowSplit {
 
 // One level-0 slab per launch, all of its work in that launch: the VJP terms
 // of slab m in every pivot-row group and (FWD) the forward of slab m + 1 from
-// slabs m and m - 1.  Block b has role b % NR (a row group, or the forward)
-// and covers chunk b / NR of the slab, so the NR blocks of one chunk run at
-// the same time and read the same slab data: each slab is fetched from HBM
-// about once per launch instead of once per row group.  The forward writes
+// slabs m and m - 1.  Logical block l (xcd_block) has role l % NR (a row
+// group, or the forward) and covers chunk l / NR of the slab, so the NR
+// blocks of one chunk run at the same time on one XCD and read the same slab
+// data: each slab is fetched about once per launch instead of once per row
+// group.  The forward writes
 // slab m + 1, which no VJP term of this launch reads.
 template <int D, bool FWD>
 __global__ __launch_bounds__(256) void slab_fused_kernel(SlabVjpArgs v, Slab0Args f) {
     using RS = RowSplit<D>;
     constexpr int NR = RS::NG + (FWD ? 1 : 0);
-    const int64_t role = blockIdx.x % NR, blk = blockIdx.x / NR, nblk = gridDim.x / NR;
+    const int64_t l = xcd_block(blockIdx.x, gridDim.x);  // the NR roles of a chunk on one XCD
+    const int64_t role = l % NR, blk = l / NR, nblk = gridDim.x / NR;
     if (role == 0) {
         vjp_slab_body<D, 0, RS::B1>(v, blk, nblk);
     } else if (RS::NG >= 2 && role == 1) {
