@@ -18,6 +18,8 @@
 //     written yet); points are visited level by level, slab by slab, which
 //     keeps every read behind the point (all reads lie below the start of
 //     the point's slab);
+//   stable forward: core.py:164-213 with every term, in flat order by one
+//     thread (exact_stable_forward);
 //   VJP: gradients.py:60-82 with every term, negative indices wrapped
 //     around the whole lattice, sums over the team in any order (a sum of
 //     values with an inf or NaN is the same in every order; finite parts
@@ -121,6 +123,48 @@ __device__ void exact_forward(const Ops &t, const Lat &g, int top, const double2
             t.sync();
         }
     }
+}
+
+// G of the all-pivot average (core.py:164-213) with every term, in the
+// reference's flat order by one thread (rank 0): a zero-weight read can
+// reach any earlier flat position, so no two points are independent a priori.
+// A negative index wraps around the lattice; it reads a value only when that
+// position is already written (earlier in flat order), else zero, as the
+// reference's zero-initialised G holds there.
+template <class Ops>
+__device__ void exact_stable_forward(const Ops &t, const Lat &g, const double2 *A, const double2 *b, double2 c, double2 *G,
+                                     const double *sq) {
+    if (t.rank == 0) {
+        const int D = g.D;
+        const int64_t N = g.N;
+        const double2 zero = make_double2(0.0, 0.0);
+        int k[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        G[0] = c;
+        for (int64_t p = 1; p < N; ++p) {
+            for (int a = D - 1; a >= 0; --a) {  // np.ndindex order
+                if (++k[a] < g.shape[a]) break;
+                k[a] = 0;
+            }
+            auto rd = [&](int64_t q) {
+                if (q < 0) q += N;
+                return (q >= 0 && q < p) ? G[q] : zero;
+            };
+            int num = 0;
+            double2 vals = zero;
+            for (int i = 0; i < D; ++i) {
+                if (k[i] == 0) continue;
+                ++num;
+                const int64_t pivot = p - g.stride[i];
+                double2 val = cmul(b[i], G[pivot]);
+                for (int j = 0; j < i; ++j) val = cadd(val, cmul(cscale_promoted(A[i * D + j], sq[k[j]]), rd(pivot - g.stride[j])));
+                val = cadd(val, cmul(cscale_promoted(A[i * D + i], sq[k[i] - 1]), rd(pivot - g.stride[i])));
+                for (int j = i + 1; j < D; ++j) val = cadd(val, cmul(cscale_promoted(A[i * D + j], sq[k[j]]), rd(pivot - g.stride[j])));
+                vals = cadd(vals, cdiv_promoted(val, sq[k[i]]));
+            }
+            G[p] = cdiv_promoted(vals, double(num));
+        }
+    }
+    t.sync();
 }
 
 // dL/dA, dL/db, dL/dc of gradients.py:41-82, one accumulator at a time.

@@ -414,6 +414,7 @@ struct StableArgs {
     const double *sq, *rsq;
     int n;                 // wavefront (per-wavefront launch)
     int64_t tstride_a, tstride_b, tstride_g;  // batch strides (complex elements)
+    int *nf;               // per-wavefront launches: raised when a stored value is not finite
 };
 
 __global__ void row_table_kernel(Geom g, uint16_t *rows, int64_t nrows) {
@@ -434,8 +435,9 @@ __global__ void row_table_kernel(Geom g, uint16_t *rows, int64_t nrows) {
     }
 }
 
+// Returns true when the stored value is not finite.
 template <int D>
-__device__ __forceinline__ void stable_point(const StableArgs &a, const double2 *A, const double2 *b, double2 *G, const int (&k)[D],
+__device__ __forceinline__ bool stable_point(const StableArgs &a, const double2 *A, const double2 *b, double2 *G, const int (&k)[D],
                                              int64_t f) {
     int num = 0;
     double2 vals = make_double2(0.0, 0.0);
@@ -457,11 +459,13 @@ __device__ __forceinline__ void stable_point(const StableArgs &a, const double2 
         }
         vals = cadd(vals, cdiv_by(val, a.sq[k[i]], a.rsq[k[i]]));
     }
-    G[f] = cdiv_py(vals, make_double2(double(num), 0.0));
+    const double2 out = cdiv_py(vals, make_double2(double(num), 0.0));
+    G[f] = out;
+    return !finite2(out);
 }
 
 template <int D>
-__device__ __forceinline__ void stable_row(const StableArgs &a, const double2 *A, const double2 *b, double2 *G, int64_t r, int n) {
+__device__ __forceinline__ bool stable_row(const StableArgs &a, const double2 *A, const double2 *b, double2 *G, int64_t r, int n) {
     const uint16_t *rd = a.rows + r * 8;
     int k[D];
     int p = 0;
@@ -471,16 +475,18 @@ __device__ __forceinline__ void stable_row(const StableArgs &a, const double2 *A
         p += k[j];
     }
     const int kl = n - p;
-    if (kl < 0 || kl >= a.g.shape[D - 1]) return;
+    if (kl < 0 || kl >= a.g.shape[D - 1]) return false;
     k[D - 1] = kl;
-    stable_point<D>(a, A, b, G, k, r * a.g.shape[D - 1] + kl);
+    return stable_point<D>(a, A, b, G, k, r * a.g.shape[D - 1] + kl);
 }
 
 // One wavefront of one triple per launch (large lattices).
 template <int D>
 __global__ __launch_bounds__(256) void stable_wavefront_kernel(StableArgs a) {
     const int64_t r0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (int64_t r = r0; r < a.nrows; r += int64_t(gridDim.x) * blockDim.x) stable_row<D>(a, a.A, a.b, a.G, r, a.n);
+    bool bad = false;
+    for (int64_t r = r0; r < a.nrows; r += int64_t(gridDim.x) * blockDim.x) bad |= stable_row<D>(a, a.A, a.b, a.G, r, a.n);
+    if (__builtin_expect(bad, 0) && a.nf) atomicOr(a.nf, 1);
 }
 
 // Whole sweep of one triple per workgroup (small lattices, batches).
@@ -494,13 +500,31 @@ __global__ __launch_bounds__(1024) void stable_block_kernel(StableArgs a) {
     __syncthreads();
     int nmax = 0;
     for (int j = 0; j < D; ++j) nmax += a.g.shape[j] - 1;
+    bool bad = !finite2(a.c[t]);
     for (int n = 1; n <= nmax; ++n) {
-        for (int64_t r = threadIdx.x; r < a.nrows; r += blockDim.x) stable_row<D>(a, A, b, G, r, n);
+        for (int64_t r = threadIdx.x; r < a.nrows; r += blockDim.x) bad |= stable_row<D>(a, A, b, G, r, n);
         __syncthreads();
+    }
+    // non-finite values: the reference's sequential sweep with every term
+    if (__syncthreads_or(bad)) {
+        __shared__ double2 red[16];
+        Lat lat;
+        lat.D = D;
+        for (int j = 0; j < 8; ++j) {
+            lat.shape[j] = a.g.shape[j];
+            lat.stride[j] = a.g.stride[j];
+        }
+        lat.N = a.g.N;
+        exact_stable_forward(BlockOps(red), lat, A, b, a.c[t], G, a.sq);
     }
 }
 
-__global__ void set_first_kernel(double2 *G, const double2 *c) { G[0] = c[0]; }
+// G[0] = c of a per-wavefront sweep; the first one of a call also clears the
+// call's non-finite flag.
+__global__ void set_first_kernel(double2 *G, const double2 *c, int *nf) {
+    G[0] = c[0];
+    if (nf) *nf = 0;
+}
 
 // ------------------------------------------------ non-finite values (exact.hpp)
 // Last launch of a multi-launch vanilla path: one workgroup per triple
@@ -514,7 +538,7 @@ struct FixArgs {
     const double *sq;
     int64_t ua, ub, uc, ug;  // per-triple strides (complex elements)
     const int *nf;           // forward: the path's flag, or NULL
-    int fwd, vjp;
+    int fwd, vjp, stable;
 };
 
 __global__ __launch_bounds__(1024) void exact_fixup_kernel(FixArgs a) {
@@ -530,7 +554,8 @@ __global__ __launch_bounds__(1024) void exact_fixup_kernel(FixArgs a) {
     if (!s_bad) return;
     const BlockOps ops(red);
     const double2 c = a.c[t * a.uc];
-    if (a.fwd) exact_forward(ops, a.g, 0, a.A + t * a.ua, a.b + t * a.ub, c, a.G + t * a.ug, a.sq);
+    if (a.fwd && a.stable) exact_stable_forward(ops, a.g, a.A + t * a.ua, a.b + t * a.ub, c, a.G + t * a.ug, a.sq);
+    else if (a.fwd) exact_forward(ops, a.g, 0, a.A + t * a.ua, a.b + t * a.ub, c, a.G + t * a.ug, a.sq);
     if (a.vjp) exact_vjp(ops, a.g, a.G + t * a.ug, a.gg + t * a.ug, c, a.dA + t * a.ua, a.db + t * a.ub, a.dc + t * a.uc, a.sq);
 }
 

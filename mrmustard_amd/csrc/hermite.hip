@@ -335,7 +335,7 @@ Lat lat_of(const Shape &S) {
 // in the reference's arithmetic when the path met a non-finite value (the
 // forward's flag nf, or a non-finite dc when the VJP was computed).
 int launch_fixup(const Shape &S, int64_t B, const double *A, const double *b, const double *c, const double *g, double *G, double *dA,
-                 double *db, double *dc, const int *nf, bool fwd, bool vjp, const DevState *ds, hipStream_t s) {
+                 double *db, double *dc, const int *nf, bool fwd, bool vjp, const DevState *ds, hipStream_t s, bool stable = false) {
     if (B < 1) return MM_OK;
     if (B > 0x7fffffff) return fail(MM_ERR_UNSUPPORTED, "too many triples for one launch");
     FixArgs a;
@@ -357,6 +357,7 @@ int launch_fixup(const Shape &S, int64_t B, const double *A, const double *b, co
     a.nf = nf;
     a.fwd = fwd;
     a.vjp = vjp;
+    a.stable = stable;
     hipLaunchKernelGGL(exact_fixup_kernel, dim3(unsigned(B)), dim3(1024), 0, s, a);
     LAUNCH_CHECK("exact_fixup_kernel");
     return MM_OK;
@@ -537,6 +538,12 @@ int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, c
     int nmax = 0;
     for (int j = 0; j < S.D; ++j) nmax += S.s[j] - 1;
     const bool small = S.N <= (int64_t(1) << 18);
+    int *nf = nullptr;
+    if (!small) {
+        rc = nf_flag(s, &nf);
+        if (rc) return rc;
+        a.nf = nf;
+    }
     for (int64_t t = 0; t < B; ++t) {
         a.A = reinterpret_cast<const double2 *>(A) + (small ? 0 : t * a.tstride_a);
         a.b = reinterpret_cast<const double2 *>(b) + (small ? 0 : t * a.tstride_b);
@@ -552,7 +559,7 @@ int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, c
             LAUNCH_CHECK("stable_block_kernel");
             break;  // one launch covers the whole batch
         }
-        hipLaunchKernelGGL(set_first_kernel, dim3(1), dim3(1), 0, s, a.G, a.c);
+        hipLaunchKernelGGL(set_first_kernel, dim3(1), dim3(1), 0, s, a.G, a.c, t == 0 ? nf : nullptr);
         LAUNCH_CHECK("set_first_kernel");
         int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((a.nrows + 255) / 256, int64_t(ds->num_cu) * 16));
         for (int n = 1; n <= nmax; ++n) {
@@ -566,6 +573,8 @@ int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, c
             LAUNCH_CHECK("stable_wavefront_kernel");
         }
     }
+    if (!small)  // non-finite values: the reference's sequential sweep of each flagged call's triples
+        return launch_fixup(S, B, A, b, c, nullptr, G, nullptr, nullptr, nullptr, nf, true, false, ds, s, true);
     return MM_OK;
 }
 

@@ -189,3 +189,36 @@ def python_complex_vjp(G, c, g):
     dc = s / complex(c)
     out = np.array([[(dLdA[i][j] + dLdA[j][i]) / 2 for j in range(D)] for i in range(D)], dtype=np.complex128)
     return out, np.array(dLdb, dtype=np.complex128), complex(dc)
+
+
+def python_complex_stable(A, b, c, shape) -> np.ndarray:
+    """stable_numba (core.py:164-213) in Python complex numbers on a list; the
+    list starts at zero as the reference's G does, and negative indices wrap
+    as numpy's do (modulo N where numba would read before the array)."""
+    A = [[complex(x) for x in row] for row in np.asarray(A, dtype=np.complex128)]
+    b = [complex(x) for x in np.asarray(b, dtype=np.complex128)]
+    D = len(shape)
+    st = _strides(shape)
+    N = int(np.prod(shape))
+    SQRT = [pymath.sqrt(n) for n in range(max(shape) + 2)]
+    G = [0j] * N
+    G[0] = complex(c)
+    Gw = lambda x: G[x % N]  # noqa: E731
+    for flat, nd in enumerate(itertools.product(*[range(s) for s in shape])):
+        if flat == 0:
+            continue
+        num, vals = 0, 0
+        for i in range(D):
+            if nd[i] == 0:
+                continue
+            num += 1
+            pivot = flat - st[i]
+            val = b[i] * G[pivot]
+            for j in range(i):
+                val += A[i][j] * SQRT[nd[j]] * Gw(pivot - st[j])
+            val += A[i][i] * SQRT[nd[i] - 1] * Gw(pivot - st[i])
+            for j in range(i + 1, D):
+                val += A[i][j] * SQRT[nd[j]] * Gw(pivot - st[j])
+            vals += val / SQRT[nd[i]]
+        G[flat] = vals / num
+    return np.array(G, dtype=np.complex128).reshape(shape)

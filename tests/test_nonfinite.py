@@ -20,7 +20,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from tests.brute import python_complex_vanilla, python_complex_vjp
+from tests.brute import python_complex_stable, python_complex_vanilla, python_complex_vjp
 from tests.conftest import gpu_available
 
 TOL = 1e-12
@@ -101,6 +101,15 @@ def test_oracle_poisoned_cotangent_matches_python_complex(oracle_mod, shape):
     g = poisoned_cotangent(np.random.default_rng(sum(shape)), shape)
     for x, r in zip(oracle_mod.vanilla_vjp(G, c, g), python_complex_vjp(G, c, g)):
         assert same_values(x, r)
+
+
+@pytest.mark.parametrize("shape,sb", [((9, 8), 1e60), ((5, 4, 4), 1e80), ((1, 6, 5), 1e70), ((3, 3, 3, 3), 1e100)],
+                         ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else f"{v:g}")
+def test_oracle_stable_blowup_matches_python_complex(oracle_mod, shape, sb):
+    A, b, c = blowup_triple(sum(shape) + 1, len(shape), sb, sA=1e40)
+    G = oracle_mod.stable(shape, A, b, c)
+    assert has_mix(G)
+    assert same_values(G, python_complex_stable(A, b, c, shape))
 
 
 def test_zero_weight_terms_matter(oracle_mod):
@@ -256,3 +265,29 @@ def test_blowup_plan(oracle_mod, st, fused):
         for k, (got, ref) in enumerate(zip(plan.split_grads(*out[1:]), grads_ref)):
             for x, r in zip(got, ref):
                 assert same_class_close(np_(x), r), (k, shapes[k])
+
+
+@pytest.mark.parametrize("shape,sb", [((60, 60), 1e6), ((30, 30, 30), 1e6), ((20, 20), 1e10), ((600, 500), 1e3)],
+                         ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else f"{v:g}")
+@pytest.mark.gpu
+@needs_gpu
+def test_stable_blowup(st, oracle_mod, shape, sb):
+    """The all-pivot average on overflowing lattices: one workgroup per triple
+    (up to 2^18 entries) and one launch per index-sum wavefront (above), then
+    the reference's sequential sweep where a non-finite value appeared."""
+    A, b, c = blowup_triple(sum(shape) + 5, len(shape), sb)
+    ref = oracle_mod.stable(shape, A, b, c)
+    assert has_mix(ref)
+    print(f"stable blow-up shape={shape}", flush=True)
+    assert same_values(np_(st.stable_hip(shape, A, b, c)), ref)
+
+
+@pytest.mark.gpu
+@needs_gpu
+def test_stable_blowup_batch(st, oracle_mod):
+    shape = (30, 30, 30)
+    A, b, c = _batch(4, shape, {2}, 1e6, 77)
+    ref = oracle_mod.vanilla_batch(shape, A, b, c, stable=True)
+    assert has_mix(ref[2])
+    print("stable blow-up batch", flush=True)
+    assert same_values(np_(st.vanilla_batch_hip(shape, A, b, c, stable=True)), ref)
