@@ -84,6 +84,63 @@ def traffic_from_profiles(name: str):
     return None
 
 
+def counter_scale():
+    """FETCH_SIZE / WRITE_SIZE scale factors measured on a 1 GiB device copy
+    (recorded with the committed traffic summary), else 2 and 1."""
+    path = os.path.join(ROOT, "profiles", "traffic_fused_batch_30x30x30.txt")
+    if os.path.exists(path):
+        import re
+
+        m = re.search(r"FETCH_SIZE is scaled by ([\d.]+) and WRITE_SIZE by ([\d.]+)", open(path).read())
+        if m:
+            return float(m.group(1)), float(m.group(2))
+    return 2.0, 1.0
+
+
+def live_traffic(kernel_prefix: str = "void mm::lds_kernel<3, true, 512>", timeout_s: int = 240):
+    """HBM bytes per launch of the headline kernel in this run: two rocprofv3
+    counter passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --probe
+    headline` in child processes, scaled as calibrated on a plain copy.
+    Returns (bytes, note); bytes is None when rocprofv3 is unavailable or a
+    pass fails."""
+    import shutil
+    import signal
+    import sqlite3
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    ff, wf = counter_scale()
+    kib = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="mm_pmc_")
+        cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__), "--probe", "headline"]
+        proc = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, start_new_session=True,
+                                env=dict(os.environ, TMPDIR="/tmp"))
+        try:
+            proc.wait(timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.wait()
+            return None, f"rocprofv3 --pmc {ctr} timed out"
+        dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
+        if not dbs:
+            return None, f"rocprofv3 --pmc {ctr} wrote no database (exit {proc.returncode})"
+        con = sqlite3.connect(dbs[0])
+        vals = [v for name, v in con.execute(
+            "select kernel_name, sum(value) from counters_collection where counter_name = ? group by dispatch_id", (ctr,))
+            if name.startswith(kernel_prefix)]
+        con.close()
+        if not vals:
+            return None, f"no {ctr} record of the headline kernel"
+        kib[ctr] = sum(vals) / len(vals)
+    traffic = (kib["FETCH_SIZE"] * ff + kib["WRITE_SIZE"] * wf) * 1024.0
+    return traffic, (f"measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over bench.py --probe headline, "
+                     f"{kib['FETCH_SIZE']:.0f} / {kib['WRITE_SIZE']:.0f} KiB per launch, scaled by {ff:.3f} / {wf:.3f}")
+
+
 def run_probe(which: str, dev) -> None:
     """Run one workload three times without timing (kernel traces and counters)."""
     import torch
@@ -134,6 +191,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--quick", action="store_true", help="headline measurement only (for profiling)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-live-traffic", action="store_true",
+                    help="take roofline.traffic from the committed counter summary instead of two rocprofv3 passes in this run")
     ap.add_argument(
         "--copy-probe",
         action="store_true",
@@ -253,6 +312,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profiles("traffic_fused_batch_30x30x30.txt"),
+        "traffic_source": "profiles/traffic_fused_batch_30x30x30.txt",
         "kernel": "lds_kernel<3, true, 512> (uniform mode, one workgroup per triple)",
         "fp64_tflops": round(flops_per_launch / t_fused / 1e12, 2),
         "fp64_frac": round(flops_per_launch / t_fused / 1e12 / FP64_PEAK_TFLOPS, 4),
@@ -517,6 +577,16 @@ def main():
             "note": "one triple; fused = mm_hermite_fwd_contract (three-slab window, each slab contracted as made); "
                     "unfused = whole lattice, then a product on the matrix cores",
         }
+
+    if not args.quick and not args.no_live_traffic:
+        log("HBM traffic of the headline kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE passes in child processes")
+        torch.cuda.synchronize()
+        t_bytes, note = live_traffic()
+        if t_bytes is not None:
+            roofline["traffic"] = round(t_bytes, 1)
+            roofline["traffic_source"] = note
+        else:
+            roofline["traffic_source"] += f" (live measurement unavailable: {note})"
 
     cpu = None
     parity = None
