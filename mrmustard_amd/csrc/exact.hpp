@@ -90,11 +90,13 @@ struct BlockOps {
 
 __device__ __forceinline__ int lat_digit(const Lat &g, int64_t p, int j) { return int((p / g.stride[j]) % g.shape[j]); }
 
-// G of the sub-lattice {k_0 .. k_{top-1} = 0} (the whole lattice for top = 0).
+// G of the sub-lattice {k_0 .. k_{top-1} = 0} (the whole lattice for top = 0);
+// row i of A starts at A + i * lda (lda = 0: D).
 template <class Ops>
 __device__ void exact_forward(const Ops &t, const Lat &g, int top, const double2 *A, const double2 *b, double2 c, double2 *G,
-                              const double *sq) {
+                              const double *sq, int lda = 0) {
     const int D = g.D;
+    const int ld = lda > 0 ? lda : D;
     const double2 zero = make_double2(0.0, 0.0);
     if (t.rank == 0) G[0] = c;
     t.sync();
@@ -102,7 +104,7 @@ __device__ void exact_forward(const Ops &t, const Lat &g, int top, const double2
         const int64_t sL = g.stride[L];
         const double2 bL = b[L];
         for (int k = 1; k < g.shape[L]; ++k) {
-            const double2 wLL = cscale_promoted(A[L * D + L], sq[k - 1]);
+            const double2 wLL = cscale_promoted(A[L * ld + L], sq[k - 1]);
             const double sk = sq[k];
             for (int64_t r = t.rank; r < sL; r += t.size) {
                 const int64_t p = int64_t(k) * sL + r;
@@ -116,7 +118,7 @@ __device__ void exact_forward(const Ops &t, const Lat &g, int top, const double2
                     rem -= kj * g.stride[j];
                     const int64_t qj = p - sL - g.stride[j];
                     const double2 gj = G[qj >= 0 ? qj : 0];
-                    v = cadd(v, cmul(cscale_promoted(A[L * D + j], sq[kj]), qj >= 0 ? gj : zero));
+                    v = cadd(v, cmul(cscale_promoted(A[L * ld + j], sq[kj]), qj >= 0 ? gj : zero));
                 }
                 G[p] = cdiv_promoted(v, sk);
             }

@@ -93,6 +93,118 @@ __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
     if (__builtin_expect(bad, 0) && a.nf) atomicOr(a.nf, 1);
 }
 
+// --------------------------------------------- several slabs per launch
+// Slabs m0 .. m0 + T - 1 of level d in one launch (T = 1 or 2), for slabs
+// small enough that a launch per slab costs more than the slab's work: one
+// block per CU, 1024 threads, packed digits and an LDS coefficient table
+// (32-bit index arithmetic; fwd_slab_kernel takes any slab size).  A point
+// of slab m reads slab m - 1 at most h = stride[d+1] points back (k - 1_d - 1_j, j > d)
+// and slab m - 2 at itself, so block b, owning the points [a_b, e_b) of every
+// slab, computes slab m0 + s (phase s) on [a_b - (T-1-s) h, e_b): its own
+// points and those its later phases read.  A phase's values stay in LDS for
+// the next phase (two buffers); a point's own values of the two slabs below
+// stay in the registers of the thread that owns the point in every phase
+// (points are numbered from e_b down, so a region that shrinks from the
+// front keeps the numbering).  Only owned points are written to G, each by
+// the expression of fwd_slab_kernel, so G is bit for bit the same.
+constexpr int MULTI_THREADS = 1024;
+constexpr int MULTI_KMAX = 4;  // points per thread: regions of at most 4096 points
+
+struct SlabMultiArgs {
+    Geom g;
+    const double2 *A, *b;
+    double2 *G;
+    const double *sq, *rsq;
+    int *nf;  // raised when a stored value is not finite
+    int d, m0, nphase;
+    int tile, halo, cap;    // owned points per block, neighbour reach, points per LDS buffer
+    int smax;               // row length of the coefficient table (largest shape of the axes > d)
+    int kbit[8], kmask[8];  // packed digits of the axes > d
+};
+
+template <int D>
+__global__ __launch_bounds__(MULTI_THREADS) void fwd_slab_multi_kernel(SlabMultiArgs a) {
+    extern __shared__ __align__(16) char lds[];
+    double2 *buf0 = reinterpret_cast<double2 *>(lds);
+    double2 *buf1 = buf0 + a.cap;
+    double2 *ctab = buf1 + a.cap;  // A_dj sqrt(k_j) for j > d, row j - d - 1
+    const double2 zero = make_double2(0.0, 0.0);
+    const int d = a.d, T = a.nphase, h = a.halo;
+    const int Sd = int(a.g.stride[d]);
+    for (int t = threadIdx.x; t < (D - 1 - d) * a.smax; t += blockDim.x) {
+        const int j = d + 1 + t / a.smax, k = t % a.smax;
+        ctab[t] = k < a.g.shape[j] ? cscale_promoted(a.A[d * D + j], a.sq[k]) : zero;
+    }
+    const int ab = int(xcd_block(blockIdx.x, gridDim.x)) * a.tile;
+    const int eb = min(ab + a.tile, Sd);
+    const int lo0 = max(0, ab - (T - 1) * h);
+    int st[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) st[j] = int(a.g.stride[j]);
+    uint32_t kp[MULTI_KMAX];
+    double2 p1v[MULTI_KMAX], p2v[MULTI_KMAX];
+    const int64_t base0 = int64_t(a.m0) * Sd;
+#pragma unroll
+    for (int r = 0; r < MULTI_KMAX; ++r) {
+        const int q = eb - 1 - int(threadIdx.x) - r * MULTI_THREADS;
+        kp[r] = 0;
+        p1v[r] = zero;
+        p2v[r] = zero;
+        if (q >= lo0) {
+            int rem = q;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                if (j <= d) continue;
+                const int kj = rem / st[j];
+                rem -= kj * st[j];
+                kp[r] |= uint32_t(kj) << a.kbit[j];
+            }
+            p1v[r] = a.G[base0 - Sd + q];
+            if (a.m0 >= 2) p2v[r] = a.G[base0 - 2 * Sd + q];
+        }
+    }
+    __syncthreads();  // the coefficient table
+    const double2 bd = a.b[d];
+    bool bad = false;
+    for (int s = 0; s < T; ++s) {
+        const int m = a.m0 + s;
+        const int lo = max(0, ab - (T - 1 - s) * h);
+        const int plo = max(0, ab - (T - s) * h);  // origin of the previous phase's buffer
+        const double2 *prev = (s & 1) ? buf0 : buf1;
+        double2 *cur = (s & 1) ? buf1 : buf0;
+        const double2 *G1 = a.G + int64_t(m - 1) * Sd;  // slab m - 1 in G (phase 0)
+        const double2 cdd = cscale_promoted(a.A[d * D + d], a.sq[m > 0 ? m - 1 : 0]);
+        const double sm = a.sq[m], rm = a.rsq[m];
+#pragma unroll
+        for (int r = 0; r < MULTI_KMAX; ++r) {
+            const int q = eb - 1 - int(threadIdx.x) - r * MULTI_THREADS;
+            if (q < lo) continue;
+            double2 v = cmul(bd, p1v[r]);
+            if (m >= 2) v = cadd(v, cmul(cdd, p2v[r]));
+#pragma unroll
+            for (int j = 1; j < D; ++j) {
+                if (j <= d) continue;
+                const int kj = int((kp[r] >> a.kbit[j]) & uint32_t(a.kmask[j]));
+                if (kj > 0) {
+                    const int x = q - st[j];
+                    const double2 nbv = s == 0 ? G1[x] : prev[x - plo];
+                    v = cadd(v, cmul(ctab[(j - d - 1) * a.smax + kj], nbv));
+                }
+            }
+            const double2 out = cdiv_by(v, sm, rm);
+            if (s + 1 < T) cur[q - lo] = out;
+            if (q >= ab) {
+                a.G[int64_t(m) * Sd + q] = out;
+                bad = bad || !finite2(out);
+            }
+            p2v[r] = p1v[r];
+            p1v[r] = out;
+        }
+        __syncthreads();
+    }
+    if (__builtin_expect(bad, 0) && a.nf) atomicOr(a.nf, 1);
+}
+
 // Level-0 slab m with the three slabs in separate buffers (a rolling window
 // that never holds the whole lattice): cur <- f(p1, p2).
 struct Slab0Args {

@@ -283,6 +283,45 @@ int launch_lds(const LdsArgs &a, int D, bool fused, int grid, int threads, int64
     return MM_OK;
 }
 
+// The sub-lattice {k_0..k_{top-1} = 0} of triple t (offsets in the uniform
+// strides of S), swept in LDS by one workgroup.  With two or three indices
+// left and room for the whole-lattice register path, it is swept as the
+// lattice of the sub-triple (A[top:, top:], b[top:], c), the A rows read with
+// the stride of the full triple: a point with k_<top = 0 takes its pivot and
+// every nonzero term from the axes >= top, the sub-lattice's C-order strides
+// are those of S, and so the values are the same, from the straight-line
+// D <= 3 sweep.  A non-finite value is flagged through at.P.nf (or rerun
+// in the sub-triple's arithmetic when the caller keeps no flag).
+int launch_face(LdsArgs at, const Shape &S, int top, int T, int64_t t, const DevState *ds, hipStream_t s) {
+    const int Ds = S.D - top;
+    at.n = 1;
+    if (top >= 1 && Ds >= 2 && Ds <= 3) {
+        Shape Sub;
+        Sub.D = Ds;
+        for (int i = 0; i < Ds; ++i) {
+            Sub.s[i] = S.s[top + i];
+            Sub.st[i] = S.st[top + i];
+        }
+        Sub.N = S.st[top - 1];
+        int Ts = 64;
+        if (choose_top(Sub, false, ds->max_lds, lds_maxt(Ds, false), &Ts) == 0) {
+            at.tmpl = make_job(Sub, 0, KIND_BLOCK);
+            at.tmpl.lda = S.D;
+            at.tmpl.a_off = t * S.D * S.D + int64_t(top) * S.D + top;
+            at.tmpl.b_off = t * S.D + top;
+            at.tmpl.c_off = t;
+            at.tmpl.g_off = t * S.N;
+            return launch_lds(at, Ds, false, 1, Ts, lds_bytes(Sub, 0, false, Ts / MM_WAVE, kernel_dmax(Ds)), s);
+        }
+    }
+    at.tmpl = make_job(S, top, KIND_BLOCK);
+    at.tmpl.a_off = t * S.D * S.D;
+    at.tmpl.b_off = t * S.D;
+    at.tmpl.c_off = t;
+    at.tmpl.g_off = t * S.N;
+    return launch_lds(at, S.D, false, 1, T, lds_bytes(S, top, false, T / MM_WAVE, kernel_dmax(S.D)), s);
+}
+
 Bases bases(const double *A, const double *b, const double *c, const double *g, double *G, double *dA, double *db, double *dc,
             double *partial) {
     Bases P;
@@ -314,6 +353,109 @@ int launch_slab(const SlabArgs &a, int D, int64_t Sd, int num_cu, hipStream_t s)
             return fail(MM_ERR_INVALID, "bad D");
     }
     LAUNCH_CHECK("fwd_slab_kernel");
+    return MM_OK;
+}
+
+// Slabs of level d of at most 2^17 points, whose launch costs more than
+// their work, go through fwd_slab_multi_kernel: the slab cut into one tile
+// per CU, two slabs per launch when a block's first region (tile + halo)
+// stays within 4096 points and two region buffers within the LDS, else one.
+// Measured on the (40,)^4 density matrix (39 level-0 slabs of 64,000 points):
+// one slab per launch of fwd_slab_kernel 246 us for the whole forward, of
+// this kernel 224 us, two per launch 212 us, three 220 us (a block's
+// redundant halo points cost about as much as the launches they save).
+struct MultiPlan {
+    int nphase = 0, tile = 0, cap = 0, nblk = 0, smax = 1;
+    int64_t shmem = 0;
+};
+
+bool multi_plan(const Shape &S, int d, const DevState *ds, MultiPlan *p) {
+    if (d + 1 >= S.D || S.s[d] < 2) return false;
+    const int64_t Sd = S.st[d];
+    if (Sd > (int64_t(1) << 17)) return false;
+    const int64_t h = S.st[d + 1];
+    int bits = 0, smax = 1;
+    for (int j = d + 1; j < S.D; ++j) {
+        int w = 0;
+        while ((1 << w) < S.s[j]) ++w;
+        bits += w;
+        smax = std::max(smax, int(S.s[j]));
+    }
+    if (bits > 32) return false;
+    const int64_t tile = std::max<int64_t>(64, (Sd + ds->num_cu - 1) / ds->num_cu);
+    for (int T = 2; T >= 1; --T) {
+        const int64_t cap = std::min<int64_t>(Sd, tile + (T - 1) * h);
+        const int64_t shmem = 2 * cap * 16 + int64_t(S.D - 1 - d) * smax * 16;
+        if (cap > int64_t(MULTI_THREADS) * MULTI_KMAX || shmem > ds->max_lds) continue;
+        p->nphase = T;
+        p->tile = int(tile);
+        p->cap = int(cap);
+        p->nblk = int((Sd + tile - 1) / tile);
+        p->smax = smax;
+        p->shmem = shmem;
+        return true;
+    }
+    return false;
+}
+
+int launch_slab_multi(const SlabArgs &sa, const Shape &S, const MultiPlan &p, int d, int m0, int T, hipStream_t s) {
+    SlabMultiArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = sa.g;
+    a.A = sa.A;
+    a.b = sa.b;
+    a.G = sa.G;
+    a.sq = sa.sq;
+    a.rsq = sa.rsq;
+    a.nf = sa.nf;
+    a.d = d;
+    a.m0 = m0;
+    a.nphase = T;
+    a.tile = p.tile;
+    a.halo = int(S.st[d + 1]);
+    a.cap = p.cap;
+    a.smax = p.smax;
+    int bit = 0;
+    for (int j = S.D - 1; j > d; --j) {
+        int w = 0;
+        while ((1 << w) < S.s[j]) ++w;
+        a.kbit[j] = bit;
+        a.kmask[j] = (1 << w) - 1;
+        bit += w;
+    }
+    const dim3 grid(unsigned(p.nblk)), block(MULTI_THREADS);
+    switch (S.D) {
+#define CASE(DD)                                                                                      \
+    case DD:                                                                                          \
+        raise_lds_limit(fwd_slab_multi_kernel<DD>, int(p.shmem));                                     \
+        hipLaunchKernelGGL((fwd_slab_multi_kernel<DD>), grid, block, size_t(p.shmem), s, a);          \
+        break;
+        CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default:
+            return fail(MM_ERR_INVALID, "bad D");
+    }
+    LAUNCH_CHECK("fwd_slab_multi_kernel");
+    return MM_OK;
+}
+
+// Slabs m = 1 .. shape[d] - 1 of level d: several per launch where
+// multi_plan allows, else one launch each.
+int launch_level(SlabArgs sa, const Shape &S, int d, const DevState *ds, hipStream_t s) {
+    MultiPlan p;
+    sa.d = d;
+    if (multi_plan(S, d, ds, &p)) {
+        for (int m0 = 1; m0 < S.s[d]; m0 += p.nphase) {
+            const int rc = launch_slab_multi(sa, S, p, d, m0, std::min(p.nphase, S.s[d] - m0), s);
+            if (rc) return rc;
+        }
+        return MM_OK;
+    }
+    for (int m = 1; m < S.s[d]; ++m) {
+        sa.m = m;
+        const int rc = launch_slab(sa, S.D, S.st[d], ds->num_cu, s);
+        if (rc) return rc;
+    }
     return MM_OK;
 }
 
@@ -659,14 +801,7 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
         at.P.g = nullptr;
         at.P.nf = nf;
         at.clear_nf = t == 0;  // the first launch of the path (one workgroup)
-        at.n = 1;
-        at.tmpl = make_job(S, top, KIND_BLOCK);
-        at.tmpl.a_off = t * a.ua;
-        at.tmpl.b_off = t * a.ub;
-        at.tmpl.c_off = t;
-        at.tmpl.g_off = t * a.ug;
-        int64_t shmem = lds_bytes(S, top, false, T / MM_WAVE, kernel_dmax(S.D));
-        int rc = launch_lds(at, S.D, false, 1, T, shmem, s);
+        int rc = launch_face(at, S, top, T, t, ds, s);
         if (rc) return rc;
         SlabArgs sa;
         sa.g = geom_of(S);
@@ -681,12 +816,8 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
         // so that each slab is read while it is still in the caches
         const bool fuse_level0 = fused && B == 1;
         for (int d = top - 1; d >= (fuse_level0 ? 1 : 0); --d) {
-            for (int m = 1; m < S.s[d]; ++m) {
-                sa.d = d;
-                sa.m = m;
-                rc = launch_slab(sa, S.D, S.st[d], ds->num_cu, s);
-                if (rc) return rc;
-            }
+            rc = launch_level(sa, S, d, ds, s);
+            if (rc) return rc;
         }
         if (fuse_level0) {
             const int64_t S0 = S.st[0];
@@ -757,9 +888,7 @@ int window_face(const Shape &S, const double *A, const double *b, const double *
     a.T.sq = ds->sq;
     a.T.rsq = ds->rsq;
     a.mode = MODE_UNIFORM;
-    a.n = 1;
-    a.tmpl = make_job(S, top, KIND_BLOCK);
-    int rc = launch_lds(a, S.D, false, 1, T, lds_bytes(S, top, false, T / MM_WAVE, kernel_dmax(S.D)), s);
+    int rc = launch_face(a, S, top, T, 0, ds, s);
     if (rc) return rc;
     SlabArgs sa;
     sa.g = geom_of(S);
@@ -770,12 +899,8 @@ int window_face(const Shape &S, const double *A, const double *b, const double *
     sa.rsq = ds->rsq;
     sa.nf = nullptr;
     for (int d = top - 1; d >= 1; --d) {
-        for (int m = 1; m < S.s[d]; ++m) {
-            sa.d = d;
-            sa.m = m;
-            rc = launch_slab(sa, S.D, S.st[d], ds->num_cu, s);
-            if (rc) return rc;
-        }
+        rc = launch_level(sa, S, d, ds, s);
+        if (rc) return rc;
     }
     return MM_OK;
 }

@@ -29,6 +29,7 @@ struct Job {
     int32_t kbit[8];   // bit offset of digit k_j in the packed per-point index word
     int32_t kmask[8];  // mask of digit k_j (after the shift)
     int64_t a_off, b_off, c_off, g_off, p_off;
+    int32_t lda;  // row stride of A (0: D); a sub-triple A[top:, top:] of a larger one has the larger D
 };
 static_assert(sizeof(Job) <= 256, "Job must fit its LDS header");
 

@@ -592,7 +592,8 @@ __device__ __forceinline__ void sweep_lds(const Bases P, const Tables T, char *l
     }
     if (tm.rank < D) bv[tm.rank] = P.b[J.b_off + tm.rank];
     if (tm.rank == 0) bv[D] = P.c[J.c_off];
-    for (int t = tm.rank; t < D * D; t += tm.size) am[t] = Ag[t];
+    const int lda = J.lda > 0 ? J.lda : D;
+    for (int t = tm.rank; t < D * D; t += tm.size) am[t] = Ag[(t / D) * lda + t % D];
     const int nrows = coef_rows(D, top);
     for (int t = tm.rank; t < nrows * smax; t += tm.size) {
         int r = t / smax, k = t - r * smax;
@@ -602,7 +603,7 @@ __device__ __forceinline__ void sweep_lds(const Bases P, const Tables T, char *l
             ++i;
         }
         int j = i + rr;
-        double2 a = Ag[i * D + j];
+        double2 a = Ag[i * lda + j];
         coef[t] = (k < J.shape[j]) ? cscale_promoted(a, T.sq[k]) : make_double2(0.0, 0.0);
     }
     for (int f = tm.rank; f < S; f += tm.size) {
@@ -770,11 +771,11 @@ __device__ __forceinline__ void sweep_lds(const Bases P, const Tables T, char *l
             const double2 *bg = P.b + J.b_off;
             if (WAVE) {
                 const WaveOps ops;
-                exact_forward(ops, lat, top, Ag, bg, c, G, T.sq);
+                exact_forward(ops, lat, top, Ag, bg, c, G, T.sq, lda);
                 if (FUSED) exact_vjp(ops, lat, G, g, c, P.dA + J.a_off, P.db + J.b_off, P.dc + J.c_off, T.sq);
             } else {
                 const BlockOps ops(red);
-                exact_forward(ops, lat, top, Ag, bg, c, G, T.sq);
+                exact_forward(ops, lat, top, Ag, bg, c, G, T.sq, lda);
                 if (FUSED) exact_vjp(ops, lat, G, g, c, P.dA + J.a_off, P.db + J.b_off, P.dc + J.c_off, T.sq);
             }
         }

@@ -53,6 +53,10 @@ FWD_SHAPES = [
     (3, 4, 5), (1, 2, 3), (30, 30, 30), (8, 1, 9), (5, 4, 3, 6), (2, 2, 2, 2, 2),
     (3, 3, 2, 2, 2, 3), (2,) * 7, (3,) * 8,
     (4, 80, 80),        # axis-0 slab too large for LDS: sub-lattice in LDS, slabs in global memory
+    (6, 40, 40, 40),    # global slabs two per launch, then one
+    (5, 3, 50, 41),     # global slabs two per launch
+    (3, 2, 70, 70),     # global slabs one per launch (the halo of a second would not fit)
+    (4, 6, 30, 30, 30), # level-1 slabs two per launch, level-0 slabs in fwd_slab_kernel
     (2, 3, 40, 45),
     (2, 4, 4, 4, 4, 4, 4, 4),
     (5, 700), (4, 20, 40), (3, 32, 32),   # level-0 slab of 512..1024 points: one point per thread
@@ -157,7 +161,7 @@ def test_vjp(st, oracle_mod, shape):
         assert rel(np_(x), rs) <= TOL
 
 
-@pytest.mark.parametrize("shape", [(23,), (9, 11), (30, 30, 30), (5, 4, 3, 6), (4, 80, 80), (6, 700), (5, 20, 40)],
+@pytest.mark.parametrize("shape", [(23,), (9, 11), (30, 30, 30), (5, 4, 3, 6), (4, 80, 80), (6, 700), (5, 20, 40), (3, 6, 30, 30, 30)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_fused(st, oracle_mod, shape):
     rng = np.random.default_rng(sum(shape) + 2000)
@@ -245,7 +249,7 @@ def test_plan_graph_replay(oracle_mod, st):
     _uneven(oracle_mod, 150, 23, True, graph=True)
 
 
-@pytest.mark.parametrize("shape", [(7,), (9, 11), (30, 30, 30), (5, 80, 80), (3, 5, 40, 45), (3, 4, 4, 4, 4, 4, 4, 4)],
+@pytest.mark.parametrize("shape", [(7,), (9, 11), (30, 30, 30), (5, 80, 80), (3, 5, 40, 45), (3, 4, 4, 4, 4, 4, 4, 4), (3, 6, 30, 30, 30)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_vjp_recompute(st, oracle_mod, shape):
     """The windowed recompute VJP equals the VJP of the stored forward."""
