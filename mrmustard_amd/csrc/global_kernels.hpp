@@ -98,17 +98,21 @@ __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
 // small enough that a launch per slab costs more than the slab's work: one
 // block per CU, 1024 threads, packed digits and an LDS coefficient table
 // (32-bit index arithmetic; fwd_slab_kernel takes any slab size).  A point
-// of slab m reads slab m - 1 at most h = stride[d+1] points back (k - 1_d - 1_j, j > d)
-// and slab m - 2 at itself, so block b, owning the points [a_b, e_b) of every
-// slab, computes slab m0 + s (phase s) on [a_b - (T-1-s) h, e_b): its own
+// of slab m reads slab m - 1 at most h = stride[d+1] points back
+// (k - 1_d - 1_j, j > d) and slab m - 2 at itself, so block b, owning the
+// points [a_b, e_b) of every slab, computes slab m0 + s (phase s) on
+// [a_b - (T-1-s) h, e_b): its own
 // points and those its later phases read.  A phase's values stay in LDS for
 // the next phase (two buffers); a point's own values of the two slabs below
 // stay in the registers of the thread that owns the point in every phase
 // (points are numbered from e_b down, so a region that shrinks from the
-// front keeps the numbering).  Only owned points are written to G, each by
-// the expression of fwd_slab_kernel, so G is bit for bit the same.
+// front keeps the numbering); phase 0 loads them and the point's neighbours
+// in slab m0 - 1 in one round trip.  Only owned points are written to G,
+// each by the expression of fwd_slab_kernel, so G is bit for bit the same;
+// a term with k_j = 0 is added as the reference adds it, a zero coefficient
+// A_dj sqrt(0) times a value read at the point itself (no branch per lane).
 constexpr int MULTI_THREADS = 1024;
-constexpr int MULTI_KMAX = 4;  // points per thread: regions of at most 4096 points
+constexpr int MULTI_KMAX = 2;  // points per thread: regions of at most 2048 points
 
 struct SlabMultiArgs {
     Geom g;
@@ -142,14 +146,17 @@ __global__ __launch_bounds__(MULTI_THREADS) void fwd_slab_multi_kernel(SlabMulti
 #pragma unroll
     for (int j = 0; j < D; ++j) st[j] = int(a.g.stride[j]);
     uint32_t kp[MULTI_KMAX];
-    double2 p1v[MULTI_KMAX], p2v[MULTI_KMAX];
+    double2 p1v[MULTI_KMAX], p2v[MULTI_KMAX], nb[MULTI_KMAX][D];
     const int64_t base0 = int64_t(a.m0) * Sd;
+    const double2 *G0 = a.G + base0 - Sd;  // slab m0 - 1
 #pragma unroll
     for (int r = 0; r < MULTI_KMAX; ++r) {
         const int q = eb - 1 - int(threadIdx.x) - r * MULTI_THREADS;
         kp[r] = 0;
         p1v[r] = zero;
         p2v[r] = zero;
+#pragma unroll
+        for (int j = 0; j < D; ++j) nb[r][j] = zero;
         if (q >= lo0) {
             int rem = q;
 #pragma unroll
@@ -158,9 +165,10 @@ __global__ __launch_bounds__(MULTI_THREADS) void fwd_slab_multi_kernel(SlabMulti
                 const int kj = rem / st[j];
                 rem -= kj * st[j];
                 kp[r] |= uint32_t(kj) << a.kbit[j];
+                nb[r][j] = G0[kj > 0 ? q - st[j] : q];  // phase 0's neighbours, loaded with the point's own values
             }
-            p1v[r] = a.G[base0 - Sd + q];
-            if (a.m0 >= 2) p2v[r] = a.G[base0 - 2 * Sd + q];
+            p1v[r] = G0[q];
+            if (a.m0 >= 2) p2v[r] = G0[q - Sd];
         }
     }
     __syncthreads();  // the coefficient table
@@ -172,7 +180,6 @@ __global__ __launch_bounds__(MULTI_THREADS) void fwd_slab_multi_kernel(SlabMulti
         const int plo = max(0, ab - (T - s) * h);  // origin of the previous phase's buffer
         const double2 *prev = (s & 1) ? buf0 : buf1;
         double2 *cur = (s & 1) ? buf1 : buf0;
-        const double2 *G1 = a.G + int64_t(m - 1) * Sd;  // slab m - 1 in G (phase 0)
         const double2 cdd = cscale_promoted(a.A[d * D + d], a.sq[m > 0 ? m - 1 : 0]);
         const double sm = a.sq[m], rm = a.rsq[m];
 #pragma unroll
@@ -185,11 +192,9 @@ __global__ __launch_bounds__(MULTI_THREADS) void fwd_slab_multi_kernel(SlabMulti
             for (int j = 1; j < D; ++j) {
                 if (j <= d) continue;
                 const int kj = int((kp[r] >> a.kbit[j]) & uint32_t(a.kmask[j]));
-                if (kj > 0) {
-                    const int x = q - st[j];
-                    const double2 nbv = s == 0 ? G1[x] : prev[x - plo];
-                    v = cadd(v, cmul(ctab[(j - d - 1) * a.smax + kj], nbv));
-                }
+                // k_j = 0: the point itself times the zero coefficient A_dj sqrt(0)
+                const double2 nbv = s == 0 ? nb[r][j] : prev[(kj > 0 ? q - st[j] : q) - plo];
+                v = cadd(v, cmul(ctab[(j - d - 1) * a.smax + kj], nbv));
             }
             const double2 out = cdiv_by(v, sm, rm);
             if (s + 1 < T) cur[q - lo] = out;

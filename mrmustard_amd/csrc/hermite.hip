@@ -359,11 +359,12 @@ int launch_slab(const SlabArgs &a, int D, int64_t Sd, int num_cu, hipStream_t s)
 // Slabs of level d of at most 2^17 points, whose launch costs more than
 // their work, go through fwd_slab_multi_kernel: the slab cut into one tile
 // per CU, two slabs per launch when a block's first region (tile + halo)
-// stays within 4096 points and two region buffers within the LDS, else one.
-// Measured on the (40,)^4 density matrix (39 level-0 slabs of 64,000 points):
-// one slab per launch of fwd_slab_kernel 246 us for the whole forward, of
-// this kernel 224 us, two per launch 212 us, three 220 us (a block's
-// redundant halo points cost about as much as the launches they save).
+// stays within 2048 points and two region buffers within the LDS, else one.
+// Measured on the (40,)^4 density matrix (39 level-0 slabs of 64,000 points),
+// the whole forward: one slab per launch of fwd_slab_kernel 246 us, of this
+// kernel 219 us, two per launch 199 us; three per launch measured no faster
+// than two (a block's redundant halo points cost about as much as the
+// launches they save).
 struct MultiPlan {
     int nphase = 0, tile = 0, cap = 0, nblk = 0, smax = 1;
     int64_t shmem = 0;
