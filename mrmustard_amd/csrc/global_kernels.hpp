@@ -78,12 +78,23 @@ __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
     bool bad = false;
     for (int64_t q = q0; q < hi; q += inc) {
         const int64_t f = base + q;
+        // every load of the point first (one round trip); a term with k_j = 0
+        // is the zero coefficient A_dj sqrt(0) times the point's own value,
+        // as the reference adds it
+        double2 nbv[D];
+        double wv[D];
+#pragma unroll
+        for (int j = 1; j < D; ++j) {
+            if (j <= d) continue;
+            nbv[j] = G[f - Sd - (k[j] > 0 ? a.g.stride[j] : 0)];
+            wv[j] = a.sq[k[j]];
+        }
         double2 v = cmul(bd, G[f - Sd]);
         if (m >= 2) v = cadd(v, cmul(cdd, G[f - 2 * Sd]));
 #pragma unroll
         for (int j = 1; j < D; ++j) {
             if (j <= d) continue;
-            if (k[j] > 0) v = cadd(v, cmul(cscale_promoted(Adj[j], a.sq[k[j]]), G[f - Sd - a.g.stride[j]]));
+            v = cadd(v, cmul(cscale_promoted(Adj[j], wv[j]), nbv[j]));
         }
         const double2 out = cdiv_by(v, sm, rm);
         G[f] = out;
@@ -101,9 +112,9 @@ __global__ __launch_bounds__(256) void fwd_slab_kernel(SlabArgs a) {
 // of slab m reads slab m - 1 at most h = stride[d+1] points back
 // (k - 1_d - 1_j, j > d) and slab m - 2 at itself, so block b, owning the
 // points [a_b, e_b) of every slab, computes slab m0 + s (phase s) on
-// [a_b - (T-1-s) h, e_b): its own
-// points and those its later phases read.  A phase's values stay in LDS for
-// the next phase (two buffers); a point's own values of the two slabs below
+// [a_b - (T-1-s) h, e_b): its own points and those its later phases read.
+// A phase's values stay in LDS for the next phase (two buffers); a point's
+// own values of the two slabs below
 // stay in the registers of the thread that owns the point in every phase
 // (points are numbered from e_b down, so a region that shrinks from the
 // front keeps the numbering); phase 0 loads them and the point's neighbours
@@ -239,11 +250,18 @@ __device__ __forceinline__ void fwd_slab0_body(const Slab0Args &a, int64_t blk, 
 #pragma unroll
     for (int j = 0; j < D; ++j) A0j[j] = (j > 0) ? a.A[j] : make_double2(0.0, 0.0);
     for (int64_t q = q0; q < hi; q += inc) {
+        // every load of the point first, zero-digit terms as in fwd_slab_kernel
+        double2 nbv[D];
+        double wv[D];
+#pragma unroll
+        for (int j = 1; j < D; ++j) {
+            nbv[j] = a.p1[q - (k[j] > 0 ? a.g.stride[j] : 0)];
+            wv[j] = a.sq[k[j]];
+        }
         double2 v = cmul(b0, a.p1[q]);
         if (m >= 2) v = cadd(v, cmul(c00, a.p2[q]));
 #pragma unroll
-        for (int j = 1; j < D; ++j)
-            if (k[j] > 0) v = cadd(v, cmul(cscale_promoted(A0j[j], a.sq[k[j]]), a.p1[q - a.g.stride[j]]));
+        for (int j = 1; j < D; ++j) v = cadd(v, cmul(cscale_promoted(A0j[j], wv[j]), nbv[j]));
         a.cur[q] = cdiv_by(v, sm, rm);
         if (q + inc < hi) digits_add<D>(k, ki, a.g, 1);
     }
