@@ -241,33 +241,45 @@ __global__ __launch_bounds__(DG_THREADS) void diag_group_kernel(DiagArgs a, int 
         }
     }
 
-    // ---- inputs, scaled by sqrt of the pivot's entries (K_l)
+    // ---- inputs, scaled by sqrt of the pivot's entries (K_l).  Everything a
+    // lane may need is loaded before any of it is used, in one round trip:
+    // the inputs of the off-diagonal pivots d >= 1 (from a read-only dummy
+    // address where the lane has none) and the three sqrt entries of its mode
+    // (every sqrt a lane uses is of p_m, p_m + 1 or p_m + 2: the lanes 2d and
+    // 2d + 1 of pivot d have m = d).
+    double2 xo[M];
+#pragma unroll
+    for (int d = 1; d < M; ++d) {
+        const double2 *src = a.A;
+        int64_t e = 0;
+        if (act && (runmask >> d & 1) && i > 2 * d && pm > 0) {
+            if (i == 2 * d + 1) {
+                src = a.arr2;
+                e = (a.off2[d] + f - sm) * nb + col;
+            } else {
+                src = (i & 1) ? a.a1010 : a.a1001;
+                e = (a.off11[d] + (m - d - 1) * a.len[d] + f - sm) * nb + col;
+            }
+        }
+        xo[d] = src[e];
+    }
+    const double s0 = a.sq[pm], s1 = a.sq[pm + 1], r1 = a.rsq[pm + 1], s2 = a.sq[pm + 2], r2 = a.rsq[pm + 2];
+    double2 bi = make_double2(0.0, 0.0);
+    if (act) bi = a.B[int64_t(src_index<M, REORDER>(i)) * nb + col];
     const bool diag = act && (a.cut[0] == 1 || p[0] < a.cut[0] - 1);
-    if (lane_ok) sgin[0][i][slot] = diag && pm > 0 ? cscale_promoted(xD, a.sq[pm]) : make_double2(0.0, 0.0);
+    if (lane_ok) sgin[0][i][slot] = diag && pm > 0 ? cscale_promoted(xD, s0) : make_double2(0.0, 0.0);
 #pragma unroll
     for (int d = 0; d < M; ++d) {
         if (!lane_ok || i < 2 * d) continue;
         double2 v = make_double2(0.0, 0.0);
         if (runmask >> d & 1) {
-            if (i == 2 * d) {
-                v = cscale_promoted(g0, a.sq[p[d] + 1]);
-            } else if (pm > 0) {
-                double2 x = xO;  // d == 0: the inbox
-                if (d > 0) {
-                    if (i == 2 * d + 1) {
-                        x = a.arr2[(a.off2[d] + f - sm) * nb + col];
-                    } else {
-                        const int64_t en = (a.off11[d] + (m - d - 1) * a.len[d] + f - sm) * nb + col;
-                        x = (i & 1) ? a.a1010[en] : a.a1001[en];
-                    }
-                }
-                v = cscale_promoted(x, a.sq[pm]);
-            }
+            if (i == 2 * d)
+                v = cscale_promoted(g0, s1);
+            else if (pm > 0)
+                v = cscale_promoted(d == 0 ? xO : xo[d], s0);  // d == 0: the inbox
         }
         sgin[1 + d][i][slot] = v;
     }
-    double2 bi = make_double2(0.0, 0.0);
-    if (act) bi = a.B[int64_t(src_index<M, REORDER>(i)) * nb + col];
     __syncthreads();
 
     // ---- diagonal pivot (diagonal_amps.py:108-150): arr1[i, p] -> slot i ^ 1 of p + e_m
@@ -275,7 +287,7 @@ __global__ __launch_bounds__(DG_THREADS) void diag_group_kernel(DiagArgs a, int 
         double2 dot = make_double2(0.0, 0.0);
 #pragma unroll
         for (int j = 0; j < G; ++j) dot = cadd(dot, cmul(sA[i][j], sgin[0][j][slot]));
-        const double2 out = cdiv_by(cadd(cmul(g0, bi), dot), a.sq[pm + 1], a.rsq[pm + 1]);
+        const double2 out = cdiv_by(cadd(cmul(g0, bi), dot), s1, r1);
         a.inD[((i ^ 1) * P + fw) * nb + col] = out;
         if (!(i & 1)) sev[m][slot] = out;
     }
@@ -285,26 +297,25 @@ __global__ __launch_bounds__(DG_THREADS) void diag_group_kernel(DiagArgs a, int 
 #pragma unroll
     for (int d = 0; d < M; ++d) {
         if (!(runmask >> d & 1) || i < 2 * d) continue;
-        const int pd = p[d];
         double2 dot = make_double2(0.0, 0.0);
 #pragma unroll
         for (int j = 2 * d; j < G; ++j) dot = cadd(dot, cmul(sA[i][j], sgin[1 + d][j][slot]));
         const double2 tt = cadd(cmul(sev[d][slot], bi), dot);
         if (i == 2 * d + 1) {
-            // arr0[p + e_d]: the output, and slot 0 of that params' inbox
-            const double2 o = cdiv_by(tt, a.sq[pd + 1], a.rsq[pd + 1]);
+            // arr0[p + e_d]: the output, and slot 0 of that params' inbox (p_d = p_m)
+            const double2 o = cdiv_by(tt, s1, r1);
             a.arr0[(f + a.stride[d]) * nb + col] = o;
             a.inO[fw * nb + col] = o;
         } else if (i == 2 * d) {
-            if (pd + 2 < a.cut[d]) {
-                const double2 o = cdiv_by(tt, a.sq[pd + 2], a.rsq[pd + 2]);
+            if (pm + 2 < cm) {
+                const double2 o = cdiv_by(tt, s2, r2);
                 if (d == 0)
                     a.inO[(P + fw) * nb + col] = o;  // slot 1 of p + e_0
                 else
                     a.arr2[(a.off2[d] + f) * nb + col] = o;
             }
         } else if (pm + 1 < cm) {
-            const double2 o = cdiv_by(tt, a.sq[pm + 1], a.rsq[pm + 1]);
+            const double2 o = cdiv_by(tt, s1, r1);
             if (d == 0) {
                 a.inO[((i ^ 1) * P + fw) * nb + col] = o;
             } else {

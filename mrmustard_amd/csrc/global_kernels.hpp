@@ -570,36 +570,66 @@ __global__ void row_table_kernel(Geom g, uint16_t *rows, int64_t nrows) {
     }
 }
 
-// Returns true when the stored value is not finite.
-template <int D>
+// Returns true when the stored value is not finite.  BF (branch-free): the
+// compiler can issue the loads of every pivot together; a term of the
+// reference whose weight is sqrt(0) (k_j = 0, or the A_ii term at k_i = 1)
+// is added as the reference adds it, the zero coefficient times a value
+// already computed (the pivot's own), and a pivot axis with k_i = 0 is
+// evaluated on the point's own address and left out of the sum and of the
+// count.  Without BF (fewer registers: the 1024-thread block kernel at
+// D >= 7) zero-weight terms and pivots are skipped, which gives the same
+// values for finite G.
+template <int D, bool BF>
 __device__ __forceinline__ bool stable_point(const StableArgs &a, const double2 *A, const double2 *b, double2 *G, const int (&k)[D],
                                              int64_t f) {
     int num = 0;
     double2 vals = make_double2(0.0, 0.0);
+    if constexpr (!BF) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            if (k[i] == 0) continue;
+            ++num;
+            const int64_t pivot = f - a.g.stride[i];
+            double2 val = cmul(b[i], G[pivot]);
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                if (j >= i) break;
+                if (k[j] > 0) val = cadd(val, cmul(cscale_promoted(A[i * D + j], a.sq[k[j]]), G[pivot - a.g.stride[j]]));
+            }
+            if (k[i] >= 2) val = cadd(val, cmul(cscale_promoted(A[i * D + i], a.sq[k[i] - 1]), G[pivot - a.g.stride[i]]));
+#pragma unroll
+            for (int j = i + 1; j < D; ++j)
+                if (k[j] > 0) val = cadd(val, cmul(cscale_promoted(A[i * D + j], a.sq[k[j]]), G[pivot - a.g.stride[j]]));
+            vals = cadd(vals, cdiv_by(val, a.sq[k[i]], a.rsq[k[i]]));
+        }
+        const double2 out = cdiv_py(vals, make_double2(double(num), 0.0));
+        G[f] = out;
+        return !finite2(out);
+    }
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-        if (k[i] == 0) continue;
-        ++num;
-        const int64_t pivot = f - a.g.stride[i];
+        const int64_t pivot = f - (k[i] > 0 ? a.g.stride[i] : 0);
         double2 val = cmul(b[i], G[pivot]);
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             if (j >= i) break;
-            if (k[j] > 0) val = cadd(val, cmul(cscale_promoted(A[i * D + j], a.sq[k[j]]), G[pivot - a.g.stride[j]]));
+            val = cadd(val, cmul(cscale_promoted(A[i * D + j], a.sq[k[j]]), G[pivot - (k[j] > 0 ? a.g.stride[j] : 0)]));
         }
-        if (k[i] >= 2) val = cadd(val, cmul(cscale_promoted(A[i * D + i], a.sq[k[i] - 1]), G[pivot - a.g.stride[i]]));
+        val = cadd(val, cmul(cscale_promoted(A[i * D + i], a.sq[k[i] > 0 ? k[i] - 1 : 0]), G[pivot - (k[i] >= 2 ? a.g.stride[i] : 0)]));
 #pragma unroll
-        for (int j = i + 1; j < D; ++j) {
-            if (k[j] > 0) val = cadd(val, cmul(cscale_promoted(A[i * D + j], a.sq[k[j]]), G[pivot - a.g.stride[j]]));
-        }
-        vals = cadd(vals, cdiv_by(val, a.sq[k[i]], a.rsq[k[i]]));
+        for (int j = i + 1; j < D; ++j)
+            val = cadd(val, cmul(cscale_promoted(A[i * D + j], a.sq[k[j]]), G[pivot - (k[j] > 0 ? a.g.stride[j] : 0)]));
+        const double2 term = cdiv_by(val, a.sq[k[i]], a.rsq[k[i]]);
+        const bool on = k[i] > 0;
+        vals = make_double2(on ? vals.x + term.x : vals.x, on ? vals.y + term.y : vals.y);
+        num += on ? 1 : 0;
     }
     const double2 out = cdiv_py(vals, make_double2(double(num), 0.0));
     G[f] = out;
     return !finite2(out);
 }
 
-template <int D>
+template <int D, bool BF>
 __device__ __forceinline__ bool stable_row(const StableArgs &a, const double2 *A, const double2 *b, double2 *G, int64_t r, int n) {
     const uint16_t *rd = a.rows + r * 8;
     int k[D];
@@ -612,7 +642,7 @@ __device__ __forceinline__ bool stable_row(const StableArgs &a, const double2 *A
     const int kl = n - p;
     if (kl < 0 || kl >= a.g.shape[D - 1]) return false;
     k[D - 1] = kl;
-    return stable_point<D>(a, A, b, G, k, r * a.g.shape[D - 1] + kl);
+    return stable_point<D, BF>(a, A, b, G, k, r * a.g.shape[D - 1] + kl);
 }
 
 // One wavefront of one triple per launch (large lattices).
@@ -620,7 +650,7 @@ template <int D>
 __global__ __launch_bounds__(256) void stable_wavefront_kernel(StableArgs a) {
     const int64_t r0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     bool bad = false;
-    for (int64_t r = r0; r < a.nrows; r += int64_t(gridDim.x) * blockDim.x) bad |= stable_row<D>(a, a.A, a.b, a.G, r, a.n);
+    for (int64_t r = r0; r < a.nrows; r += int64_t(gridDim.x) * blockDim.x) bad |= stable_row<D, true>(a, a.A, a.b, a.G, r, a.n);
     if (__builtin_expect(bad, 0) && a.nf) atomicOr(a.nf, 1);
 }
 
@@ -637,7 +667,7 @@ __global__ __launch_bounds__(1024) void stable_block_kernel(StableArgs a) {
     for (int j = 0; j < D; ++j) nmax += a.g.shape[j] - 1;
     bool bad = !finite2(a.c[t]);
     for (int n = 1; n <= nmax; ++n) {
-        for (int64_t r = threadIdx.x; r < a.nrows; r += blockDim.x) bad |= stable_row<D>(a, A, b, G, r, n);
+        for (int64_t r = threadIdx.x; r < a.nrows; r += blockDim.x) bad |= stable_row<D, (D <= 6)>(a, A, b, G, r, n);
         __syncthreads();
     }
     // non-finite values: the reference's sequential sweep with every term
