@@ -100,55 +100,75 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
         if (t == 0) need = 0;
         __syncthreads();
         if (runmask && i == 0) atomicOr(&need, runmask);
-        // every load of every pivot depends on p only: issue them all first
+        // every load of every pivot depends on p only: the addresses first,
+        // then all the loads at once (from a read-only dummy address where the
+        // lane needs none), then the arithmetic.  Every sqrt entry a lane uses
+        // is of p_m, p_m + 1 or p_m + 2 (the lanes 2d and 2d + 1 of pivot d
+        // have m = d, so p_d = p_m there).
+        const double s0 = a.sq[pm], s1 = a.sq[pm + 1], r1 = a.rsq[pm + 1], s2 = a.sq[pm + 2], r2 = a.rsq[pm + 2];
+        double2 xr[M], obr[M];
+#pragma unroll
+        for (int d = 0; d < M; ++d) {
+            const bool run = (runmask >> d & 1) && i >= 2 * d;
+            const double2 *xp = a.A, *op = a.A;
+            int64_t xe = 0, oe = 0;
+            if (run) {
+                if (i == 2 * d) {
+                    if (pm + 2 < cm) {
+                        op = d == 0 ? v.adO : v.ad2;
+                        oe = d == 0 ? P + fw : a.off2[d] + f;
+                    }
+                } else if (i == 2 * d + 1) {
+                    if (pm > 0) {
+                        xp = d == 0 ? a.inO : a.arr2;
+                        xe = d == 0 ? P + gs : a.off2[d] + f - sm;
+                    }
+                    op = v.adO;  // arr0[p + e_d]: slot 0 of its consumer
+                    oe = fw;
+                } else {
+                    const int64_t sl = a.off11[d] + (m - d - 1) * a.len[d];
+                    if (pm > 0) {
+                        xp = d == 0 ? a.inO : ((i & 1) ? a.a1010 : a.a1001);
+                        xe = d == 0 ? i * P + gs : sl + f - sm;
+                    }
+                    if (pm + 1 < cm) {
+                        op = d == 0 ? v.adO : ((i & 1) ? v.ad1001 : v.ad1010);
+                        oe = d == 0 ? (i ^ 1) * P + fw : sl + f;
+                    }
+                }
+            }
+            xr[d] = xp[xe];
+            obr[d] = op[oe];
+        }
+        // arr1[2d, p] for lane 2d (d = m), sent to p + e_d's slot 2d + 1
+        const bool gon = !(i & 1) && (runmask >> m & 1) && i < G;
+        const double2 gd = (gon ? a.inD : a.A)[gon ? (i + 1) * P + fw : 0];
+        // the diagonal pivot's own inputs and output adjoint (without the off-diagonal pivots' share)
+        const bool diag = act && (a.cut[0] == 1 || p[0] < a.cut[0] - 1);
+        const bool dwrite = diag && pm + 1 < cm && (i != 1 || p[0] + 2 < a.cut[0]);
+        const int c0p = p[0] + (m == 0 ? 1 : 0);  // the consumer p + e_m read it iff its own diagonal pivot ran
+        const bool xon = diag && pm > 0, oon = dwrite && (a.cut[0] == 1 || c0p < a.cut[0] - 1);
+        const double2 xDr = (xon ? a.inD : a.A)[xon ? i * P + gs : 0];
+        const double2 obDr = (oon ? v.adD : a.A)[oon ? (i ^ 1) * P + fw : 0];
         double2 xs[M], wvs[M];
-        double2 gd = make_double2(0.0, 0.0);  // arr1[2d, p] for lane 2d, sent to p + e_d's slot 2d + 1
 #pragma unroll
         for (int d = 0; d < M; ++d) {
             xs[d] = make_double2(0.0, 0.0);
             wvs[d] = make_double2(0.0, 0.0);
             if (!((runmask >> d & 1) && i >= 2 * d)) continue;
-            const int pd = p[d];
             if (i == 2 * d) {
-                xs[d] = cscale_promoted(g0, a.sq[pd + 1]);
-                gd = a.inD[(2 * d + 1) * P + fw];
-                if (pd + 2 < a.cut[d]) {
-                    const double2 ob = d == 0 ? v.adO[P + fw] : v.ad2[a.off2[d] + f];
-                    wvs[d] = cdiv_by(ob, a.sq[pd + 2], a.rsq[pd + 2]);
-                }
+                xs[d] = cscale_promoted(g0, s1);
+                if (pm + 2 < cm) wvs[d] = cdiv_by(obr[d], s2, r2);
             } else if (i == 2 * d + 1) {
-                if (pd > 0) xs[d] = cscale_promoted(d == 0 ? a.inO[P + gs] : a.arr2[a.off2[d] + f - sm], a.sq[pd]);
-                wvs[d] = cdiv_by(v.adO[fw], a.sq[pd + 1], a.rsq[pd + 1]);  // arr0[p + e_d]: slot 0 of its consumer
+                if (pm > 0) xs[d] = cscale_promoted(xr[d], s0);
+                wvs[d] = cdiv_by(obr[d], s1, r1);
             } else {
-                const int64_t sl = a.off11[d] + (m - d - 1) * a.len[d];
-                if (pm > 0) {
-                    double2 xr;
-                    if (d == 0)
-                        xr = a.inO[i * P + gs];
-                    else
-                        xr = (i & 1) ? a.a1010[sl + f - sm] : a.a1001[sl + f - sm];
-                    xs[d] = cscale_promoted(xr, a.sq[pm]);
-                }
-                if (pm + 1 < cm) {
-                    double2 ob;
-                    if (d == 0)
-                        ob = v.adO[(i ^ 1) * P + fw];
-                    else
-                        ob = (i & 1) ? v.ad1001[sl + f] : v.ad1010[sl + f];
-                    wvs[d] = cdiv_by(ob, a.sq[pm + 1], a.rsq[pm + 1]);
-                }
+                if (pm > 0) xs[d] = cscale_promoted(xr[d], s0);
+                if (pm + 1 < cm) wvs[d] = cdiv_by(obr[d], s1, r1);
             }
         }
-        // the diagonal pivot's own inputs and output adjoint (without the off-diagonal pivots' share)
-        const bool diag = act && (a.cut[0] == 1 || p[0] < a.cut[0] - 1);
-        const bool dwrite = diag && pm + 1 < cm && (i != 1 || p[0] + 2 < a.cut[0]);
-        double2 xD = make_double2(0.0, 0.0), obD = make_double2(0.0, 0.0);
-        if (diag && pm > 0) xD = cscale_promoted(a.inD[i * P + gs], a.sq[pm]);
-        if (dwrite) {
-            // the consumer p + e_m read it iff its own diagonal pivot ran
-            const int c0p = p[0] + (m == 0 ? 1 : 0);
-            if (a.cut[0] == 1 || c0p < a.cut[0] - 1) obD = v.adD[(i ^ 1) * P + fw];
-        }
+        const double2 xD = xon ? cscale_promoted(xDr, s0) : make_double2(0.0, 0.0);
+        const double2 obD = oon ? obDr : make_double2(0.0, 0.0);
 
         __syncthreads();
         const int need_all = need;
@@ -173,23 +193,22 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
                 double2 s = make_double2(0.0, 0.0);
 #pragma unroll
                 for (int r = 2 * d; r < G; ++r) s = cfma(sw[r][slot], sA[r][i], s);
-                const int pd = p[d];
                 if (i == 2 * d) {
-                    gbar0 = caxpy(a.sq[pd + 1], s, gbar0);
+                    gbar0 = caxpy(s1, s, gbar0);
                     double2 gb = make_double2(0.0, 0.0);  // adjoint of arr1[2d, p]
 #pragma unroll
                     for (int r = 2 * d; r < G; ++r) gb = cfma(sw[r][slot], sB[r], gb);
                     gext = gb;
                 } else if (i == 2 * d + 1) {
-                    if (pd > 0) {
-                        const double2 raw = make_double2(s.x * a.sq[pd], s.y * a.sq[pd]);
+                    if (pm > 0) {
+                        const double2 raw = make_double2(s.x * s0, s.y * s0);
                         if (d == 0)
                             v.adO[P + gs] = raw;
                         else
                             v.ad2[a.off2[d] + f - sm] = raw;
                     }
                 } else if (pm > 0) {
-                    const double2 raw = make_double2(s.x * a.sq[pm], s.y * a.sq[pm]);
+                    const double2 raw = make_double2(s.x * s0, s.y * s0);
                     if (d == 0) {
                         v.adO[i * P + gs] = raw;
                     } else {
@@ -209,7 +228,7 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
         if (dwrite) {
             double2 ob = obD;
             if (!(i & 1) && (runmask >> m & 1)) ob = cadd(ob, gext);
-            wv = cdiv_by(ob, a.sq[pm + 1], a.rsq[pm + 1]);
+            wv = cdiv_by(ob, s1, r1);
         }
         if (lane_ok) {
             sx[i][slot] = x;
@@ -224,7 +243,7 @@ __global__ __launch_bounds__(DG_THREADS) void diag_vjp_kernel(DiagVjpArgs v, int
             double2 s = make_double2(0.0, 0.0);
 #pragma unroll
             for (int r = 0; r < G; ++r) s = cfma(sw[r][slot], sA[r][i], s);
-            if (pm > 0) v.adD[i * P + gs] = make_double2(s.x * a.sq[pm], s.y * a.sq[pm]);
+            if (pm > 0) v.adD[i * P + gs] = make_double2(s.x * s0, s.y * s0);
             if (i == 0) {
 #pragma unroll
                 for (int r = 0; r < G; ++r) gdiag = cfma(sw[r][slot], sB[r], gdiag);
