@@ -304,6 +304,21 @@ int launch_face(LdsArgs at, const Shape &S, int top, int T, int64_t t, const Dev
         }
         Sub.N = S.st[top - 1];
         int Ts = 64;
+        // 1024-2048-point slabs: two points per thread of up to 1024 threads
+        // (lds_block_kernel), four waves per SIMD
+        const int T2 = pick_threads(Sub.st[0], 1024, 2);
+        if (Ds == 3 && Sub.st[0] > 1024 && T2 && packed_bits_ok(Sub, 0) &&
+            lds_bytes_rm(Sub, 0, false, T2 / MM_WAVE, 2) <= ds->max_lds) {
+            at.tmpl = make_job(Sub, 0, KIND_BLOCK);
+            at.tmpl.lda = S.D;
+            at.tmpl.a_off = t * S.D * S.D + int64_t(top) * S.D + top;
+            at.tmpl.b_off = t * S.D + top;
+            at.tmpl.c_off = t;
+            at.tmpl.g_off = t * S.N;
+            launch_lds_block2(at, dim3(1), dim3(T2), size_t(lds_bytes_rm(Sub, 0, false, T2 / MM_WAVE, 2)), s);
+            LAUNCH_CHECK("lds_block_kernel");
+            return MM_OK;
+        }
         if (choose_top(Sub, false, ds->max_lds, lds_maxt(Ds, false), &Ts) == 0) {
             at.tmpl = make_job(Sub, 0, KIND_BLOCK);
             at.tmpl.lda = S.D;

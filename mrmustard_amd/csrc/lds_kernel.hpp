@@ -134,6 +134,7 @@ __global__ __launch_bounds__(MAXT) void lds_kernel(LdsArgs a) {
 template <int D, bool FUSED, int MAXT, int RM>
 __global__ __launch_bounds__(MAXT) void lds_block_kernel(LdsArgs a) {
     extern __shared__ __align__(16) char lds[];
+    if (a.clear_nf && threadIdx.x == 0) *a.P.nf = 0;  // the sweep raises it only after barriers
     JobShift sh;
     sh.ua = a.ua;
     sh.ub = a.ub;

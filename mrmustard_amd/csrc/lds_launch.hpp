@@ -19,5 +19,7 @@ void launch_lds_8f(const LdsArgs &a, dim3 grid, dim3 block, size_t shmem, hipStr
 void launch_lds_8(const LdsArgs &a, dim3 grid, dim3 block, size_t shmem, hipStream_t s);   // <8, false, 512>
 // lds_block_kernel<D, false, 1024, 1>, D = 2 or 3 (forward only).
 void launch_lds_block(int D, const LdsArgs &a, dim3 grid, dim3 block, size_t shmem, hipStream_t s);
+// lds_block_kernel<3, false, 1024, 2>: a single D = 3 sub-lattice with 1024-2048-point slabs.
+void launch_lds_block2(const LdsArgs &a, dim3 grid, dim3 block, size_t shmem, hipStream_t s);
 
 }  // namespace mm
