@@ -144,13 +144,14 @@ __device__ __forceinline__ int lo_runmask(const LeftArgs &a, const LoLane<MT> &l
     return mask;
 }
 
-// (piv B_r + A_r0 sqrt(m) piv(m-1, n) + A_r1 sqrt(n) piv(m, n-1)) for output row r of the full A
-template <int L, bool REORDER>
-__device__ __forceinline__ double2 lo_self_terms(const LeftArgs &a, const double2 (*sA)[2 * L], int r, double2 pv, double2 pvm,
-                                                 double2 pvn, int bm, int bn) {
-    double2 t = cmul(pv, a.B[src_index<L, REORDER>(r)]);
-    if (bm > 0) t = cadd(t, cmul(sA[r][0], cscale_promoted(pvm, a.t.sq[bm])));
-    if (bn > 0) t = cadd(t, cmul(sA[r][1], cscale_promoted(pvn, a.t.sq[bn])));
+// (piv B_r + A_r0 sqrt(m) piv(m-1, n) + A_r1 sqrt(n) piv(m, n-1)) for output row r of the full A;
+// Br = B_r, sbm = sqrt(m), sbn = sqrt(n), loaded by the caller with its other inputs
+template <int L>
+__device__ __forceinline__ double2 lo_self_terms(const double2 (*sA)[2 * L], int r, double2 Br, double sbm, double sbn, double2 pv,
+                                                 double2 pvm, double2 pvn, int bm, int bn) {
+    double2 t = cmul(pv, Br);
+    if (bm > 0) t = cadd(t, cmul(sA[r][0], cscale_promoted(pvm, sbm)));
+    if (bn > 0) t = cadd(t, cmul(sA[r][1], cscale_promoted(pvn, sbn)));
     return t;
 }
 
@@ -167,22 +168,28 @@ __global__ __launch_bounds__(DG_THREADS) void lo_diag_kernel(LeftArgs a, int w, 
     const int64_t P = a.t.P, C = a.C;
     const int runmask = lo_runmask<MT>(a, l);
     const bool diag = l.act && (a.t.cut[0] == 1 || l.p[0] < a.t.cut[0] - 1);
-    double2 g0 = make_double2(0.0, 0.0), gm = g0, gn = g0, v = g0;
-    if (l.act) {
-        const int64_t base = l.gs * C + l.pos;  // inbox slot 0: arr0 block of p
-        g0 = a.t.inO[base];
-        if (l.bm > 0) gm = a.t.inO[base - a.c0];
-        if (l.bn > 0) gn = a.t.inO[base - 1];
-        if (diag && l.pm > 0) v = cscale_promoted(a.t.inD[(l.i * P + l.gs) * C + l.pos], a.t.sq[l.pm]);
-    }
+    // every input of the lane in one round trip: its block of arr0 and the
+    // two lowered entries (read at the entry itself where the index is 0),
+    // its diagonal input, B_r and the sqrt entries it uses
+    const int64_t base = l.act ? l.gs * C + l.pos : 0;  // inbox slot 0: arr0 block of p
+    const double2 g0r = a.t.inO[base];
+    const double2 gmr = a.t.inO[base - (l.bm > 0 ? a.c0 : 0)];
+    const double2 gnr = a.t.inO[base - (l.bn > 0 ? 1 : 0)];
+    const bool von = diag && l.pm > 0;
+    const double2 vr = a.t.inD[von ? (l.i * P + l.gs) * C + l.pos : 0];
+    const int r = l.lane_ok ? l.i + 2 : 2;
+    const double2 Br = a.B[src_index<L, REORDER>(r)];
+    const double s0 = a.t.sq[l.pm], s1 = a.t.sq[l.pm + 1], r1 = a.t.rsq[l.pm + 1], sbm = a.t.sq[l.bm], sbn = a.t.sq[l.bn];
+    const double2 zero = make_double2(0.0, 0.0);
+    const double2 g0 = l.act ? g0r : zero, gm = l.act && l.bm > 0 ? gmr : zero, gn = l.act && l.bn > 0 ? gnr : zero;
+    const double2 v = von ? cscale_promoted(vr, s0) : zero;
     if (l.lane_ok) sgin[l.i][l.slot] = v;
     __syncthreads();
     if (diag && l.pm + 1 < l.cm && (l.i != 1 || l.p[0] + 2 < a.t.cut[0])) {
-        const int r = l.i + 2;
-        double2 t = lo_self_terms<L, REORDER>(a, sA, r, g0, gm, gn, l.bm, l.bn);
+        double2 t = lo_self_terms<L>(sA, r, Br, sbm, sbn, g0, gm, gn, l.bm, l.bn);
 #pragma unroll
         for (int j = 0; j < G; ++j) t = cadd(t, cmul(sA[r][j + 2], sgin[j][l.slot]));
-        const double2 o = cdiv_by(t, a.t.sq[l.pm + 1], a.t.rsq[l.pm + 1]);
+        const double2 o = cdiv_by(t, s1, r1);
         a.t.inD[((l.i ^ 1) * P + l.fw) * C + l.pos] = o;
         if (!(l.i & 1) && (runmask >> l.m & 1)) a.piv[(l.m * P + l.gs) * C + l.pos] = o;
     }
@@ -200,28 +207,48 @@ __global__ __launch_bounds__(DG_THREADS) void lo_off_kernel(LeftArgs a, int w, i
     const LoLane<MT> l = lo_lane<MT, PB>(a, count, start, blockIdx.x);
     const int64_t P = a.t.P, C = a.C;
     const int runmask = lo_runmask<MT>(a, l);
-    double2 g0 = make_double2(0.0, 0.0);
-    if (l.act) g0 = a.t.inO[l.gs * C + l.pos];
-    // inputs of every pivot d this params runs (slots d >= 1 by flat index)
+    // every input of the lane in one round trip (from the lane's own arr0
+    // entry where it needs none): the inputs of each pivot d this params runs
+    // (slots d >= 1 by flat index), the arr1[2d, p] blocks with their two
+    // lowered entries, B_r and the sqrt entries of the lane's mode and block
+    // entry (the lanes 2d and 2d + 1 of pivot d have m = d, so p_d = p_m)
+    const int64_t own = l.act ? l.gs * C + l.pos : 0;
+    const double2 g0 = l.act ? a.t.inO[own] : make_double2(0.0, 0.0);
+    double2 xr[MT], pv[MT], pvm[MT], pvn[MT];
+#pragma unroll
+    for (int d = 0; d < MT; ++d) {
+        const double2 *xp = a.t.inO;
+        int64_t xe = own;
+        if (l.lane_ok && l.i > 2 * d && (runmask >> d & 1) && l.pm > 0) {
+            if (d == 0) {
+                xe = (l.i * P + l.gs) * C + l.pos;
+            } else if (l.i == 2 * d + 1) {
+                xp = a.t.arr2;
+                xe = (a.t.off2[d] + l.f - l.sm) * C + l.pos;
+            } else {
+                xp = (l.i & 1) ? a.t.a1010 : a.t.a1001;
+                xe = (a.t.off11[d] + (l.m - d - 1) * a.t.len[d] + l.f - l.sm) * C + l.pos;
+            }
+        }
+        xr[d] = xp[xe];
+        const int64_t pb = l.act ? (d * P + l.gs) * C + l.pos : 0;  // arr1[2d, p] block
+        pv[d] = a.piv[pb];
+        pvm[d] = a.piv[pb - (l.bm > 0 ? a.c0 : 0)];
+        pvn[d] = a.piv[pb - (l.bn > 0 ? 1 : 0)];
+    }
+    const int r = l.lane_ok ? l.i + 2 : 2;
+    const double2 Br = a.B[src_index<L, REORDER>(r)];
+    const double s0 = a.t.sq[l.pm], s1 = a.t.sq[l.pm + 1], r1 = a.t.rsq[l.pm + 1], s2 = a.t.sq[l.pm + 2], r2 = a.t.rsq[l.pm + 2];
+    const double sbm = a.t.sq[l.bm], sbn = a.t.sq[l.bn];
 #pragma unroll
     for (int d = 0; d < MT; ++d) {
         if (!l.lane_ok || l.i < 2 * d) continue;
         double2 v = make_double2(0.0, 0.0);
         if (runmask >> d & 1) {
-            if (l.i == 2 * d) {
-                v = cscale_promoted(g0, a.t.sq[l.p[d] + 1]);
-            } else if (l.pm > 0) {
-                double2 x;
-                if (d == 0) {
-                    x = a.t.inO[(l.i * P + l.gs) * C + l.pos];
-                } else if (l.i == 2 * d + 1) {
-                    x = a.t.arr2[(a.t.off2[d] + l.f - l.sm) * C + l.pos];
-                } else {
-                    const int64_t en = (a.t.off11[d] + (l.m - d - 1) * a.t.len[d] + l.f - l.sm) * C + l.pos;
-                    x = (l.i & 1) ? a.t.a1010[en] : a.t.a1001[en];
-                }
-                v = cscale_promoted(x, a.t.sq[l.pm]);
-            }
+            if (l.i == 2 * d)
+                v = cscale_promoted(g0, s1);
+            else if (l.pm > 0)
+                v = cscale_promoted(xr[d], s0);
         }
         sgin[d][l.i][l.slot] = v;
     }
@@ -229,30 +256,25 @@ __global__ __launch_bounds__(DG_THREADS) void lo_off_kernel(LeftArgs a, int w, i
 #pragma unroll
     for (int d = 0; d < MT; ++d) {
         if (!(runmask >> d & 1) || l.i < 2 * d) continue;
-        const int pd = l.p[d];
-        const int64_t pb = (d * P + l.gs) * C + l.pos;  // arr1[2d, p] block
-        const double2 pv = a.piv[pb];
-        const double2 pvm = l.bm > 0 ? a.piv[pb - a.c0] : make_double2(0.0, 0.0);
-        const double2 pvn = l.bn > 0 ? a.piv[pb - 1] : make_double2(0.0, 0.0);
-        const int r = l.i + 2;
-        double2 t = lo_self_terms<L, REORDER>(a, sA, r, pv, pvm, pvn, l.bm, l.bn);
+        const double2 zero = make_double2(0.0, 0.0);
+        double2 t = lo_self_terms<L>(sA, r, Br, sbm, sbn, pv[d], l.bm > 0 ? pvm[d] : zero, l.bn > 0 ? pvn[d] : zero, l.bm, l.bn);
 #pragma unroll
         for (int j = 2 * d; j < G; ++j) t = cadd(t, cmul(sA[r][j + 2], sgin[d][j][l.slot]));
         if (l.i == 2 * d + 1) {
             // arr0[p + e_d]: the output, and inbox slot 0 of that params
-            const double2 o = cdiv_by(t, a.t.sq[pd + 1], a.t.rsq[pd + 1]);
+            const double2 o = cdiv_by(t, s1, r1);
             a.out[int64_t(l.pos) * P + l.f + a.t.stride[d]] = o;
             a.t.inO[l.fw * C + l.pos] = o;
         } else if (l.i == 2 * d) {
-            if (pd + 2 < a.t.cut[d]) {
-                const double2 o = cdiv_by(t, a.t.sq[pd + 2], a.t.rsq[pd + 2]);
+            if (l.pm + 2 < l.cm) {
+                const double2 o = cdiv_by(t, s2, r2);
                 if (d == 0)
                     a.t.inO[(P + l.fw) * C + l.pos] = o;
                 else
                     a.t.arr2[(a.t.off2[d] + l.f) * C + l.pos] = o;
             }
         } else if (l.pm + 1 < l.cm) {
-            const double2 o = cdiv_by(t, a.t.sq[l.pm + 1], a.t.rsq[l.pm + 1]);
+            const double2 o = cdiv_by(t, s1, r1);
             if (d == 0) {
                 a.t.inO[((l.i ^ 1) * P + l.fw) * C + l.pos] = o;
             } else {
