@@ -54,6 +54,7 @@ FWD_SHAPES = [
     (3, 3, 2, 2, 2, 3), (2,) * 7, (3,) * 8,
     (4, 80, 80),        # axis-0 slab too large for LDS: sub-lattice in LDS, slabs in global memory
     (6, 40, 40, 40),    # global slabs two per launch, then one
+    (3, 50, 50, 50),    # face swept as a D = 3 sub-triple with 2,500-point slabs
     (5, 3, 50, 41),     # global slabs two per launch
     (3, 2, 70, 70),     # global slabs one per launch (the halo of a second would not fit)
     (4, 6, 30, 30, 30), # level-1 slabs two per launch, level-0 slabs in fwd_slab_kernel
