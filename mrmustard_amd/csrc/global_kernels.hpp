@@ -16,9 +16,26 @@ struct Geom {
     int64_t N;
 };
 
-// Mixed-radix digits of x over axes [lo, D).
+// Mixed-radix digits of x over axes [lo, D).  Lattices below 2^31 points use
+// 32-bit divisions (a 64-bit one is a long software sequence, and a thread of
+// a slab launch with one point per thread would spend most of its time here).
 template <int D>
 __device__ __forceinline__ void digits_of(int64_t x, const Geom &g, int lo, int (&k)[D]) {
+    if (g.N <= int64_t(0x7fffffff)) {
+        uint32_t r = uint32_t(x);
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            if (j < lo) {
+                k[j] = 0;
+                continue;
+            }
+            const uint32_t sj = uint32_t(g.stride[j]);
+            const uint32_t dj = r / sj;
+            r -= dj * sj;
+            k[j] = int(dj);
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         if (j < lo) {
