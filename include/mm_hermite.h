@@ -117,7 +117,11 @@ int mm_hermite_batch_fused(const double *A, const double *b, const double *c, co
  * the workspace, so resident memory is 3 * prod(shape[1:]) complex values
  * instead of prod(shape).  Same results as mm_hermite_vjp applied to
  * mm_hermite_fwd's G (vanilla_vjp_numba, vanilla/gradients.py:25-82, of
- * vanilla_numba's output, core.py:25-124). */
+ * vanilla_numba's output, core.py:25-124) while every value stays finite.  An
+ * inf or NaN in G or dLdG makes dc non-finite (dc is the sum of G * dLdG
+ * divided by c); the other entries may then differ from the reference's in
+ * which parts are NaN or +-inf.  A caller that needs the reference's values
+ * there calls mm_hermite_fwd and mm_hermite_vjp when dc is not finite. */
 size_t mm_hermite_vjp_recompute_workspace(const int64_t *shape, int D);
 int mm_hermite_vjp_recompute(const double *A, const double *b, const double *c, const double *dLdG,
                              const int64_t *shape, int D, double *dA, double *db, double *dc,

@@ -244,7 +244,14 @@ def vanilla_batch_fused_hip(shape, A, b, c, dLdG, out=None):
 def vanilla_vjp_recompute_hip(shape, A, b, c, dLdG):
     """(dL/dA, dL/db, dL/dc) of the forward without G: the lattice is recomputed
     slab by slab in a three-slab window (mm_hermite_vjp_recompute).  Equal to
-    ``vanilla_vjp_hip(vanilla_hip(shape, A, b, c), c, dLdG)``."""
+    ``vanilla_vjp_hip(vanilla_hip(shape, A, b, c), c, dLdG)``.
+
+    The windowed sweep does not rerun an overflowing triple in the reference's
+    arithmetic, but any inf or NaN in G or dL/dG makes dL/dc non-finite (it is
+    the sum of G * dL/dG over the lattice, divided by c).  The call therefore
+    reads dL/dc back (one stream synchronisation) and, when it is not finite,
+    takes the stored-lattice path, whose results are the reference's values,
+    NaN and +-inf included; only then is the whole lattice held."""
     L = _lib.load()
     A, b, c, g = as_device_c128(A), as_device_c128(b), as_device_c128(c), as_device_c128(dLdG)
     D = int(b.shape[-1])
@@ -261,6 +268,8 @@ def vanilla_vjp_recompute_hip(shape, A, b, c, dLdG):
         0 if ws is None else ws.numel(), _stream(),
     )
     _lib.check(rc, "mm_hermite_vjp_recompute")
+    if not bool(torch.isfinite(dc)):
+        return vanilla_vjp_hip(vanilla_hip(shape, A, b, c), c, g)
     return dA, db, dc
 
 

@@ -69,19 +69,29 @@ class HermiteRenormalizedRecompute(torch.autograd.Function):
     """As HermiteRenormalized, but autograd keeps only (A, b, c): the backward
     recomputes the lattice slab by slab in a three-slab window
     (``vanilla_vjp_recompute_hip``), so G need not stay resident between the
-    forward and the backward."""
+    forward and the backward.  The VJP applies to the forward's own lattice,
+    as the reference's backward does (jax_vjps/hermite.py:78-101): for the
+    all-pivot average the backward recomputes that lattice whole with
+    ``stable_hip`` (the windowed sweep runs the first-pivot recurrence only)
+    and frees it once the VJP is taken."""
 
     @staticmethod
     def forward(ctx, A, b, c, shape, stable):
         G = st.stable_hip(shape, A, b, c) if stable else st.vanilla_hip(shape, A, b, c)
         ctx.save_for_backward(A, b, c)
         ctx.shape = tuple(shape)
+        ctx.stable = bool(stable)
         return G
 
     @staticmethod
     def backward(ctx, grad):
         A, b, c = ctx.saved_tensors
-        dA, db, dc = st.vanilla_vjp_recompute_hip(ctx.shape, A, b, c, grad.conj())
+        if ctx.stable:
+            G = st.stable_hip(ctx.shape, A, b, c)
+            dA, db, dc = st.vanilla_vjp_hip(G, c, grad.conj())
+            del G
+        else:
+            dA, db, dc = st.vanilla_vjp_recompute_hip(ctx.shape, A, b, c, grad.conj())
         return (
             dA.conj().resolve_conj().reshape(A.shape),
             db.conj().resolve_conj().reshape(b.shape),

@@ -279,3 +279,29 @@ def test_autograd_recompute_setting(oracle_mod, st):
     torch.real(torch.sum(torch.tensor(w, device="cuda") * G)).backward()
     _, db, _ = oracle_mod.vanilla_vjp(oracle_mod.vanilla(shape, A, b, c), c, w)
     assert rel(np_(bt.grad), np.conj(db)) <= TOL
+
+
+def test_autograd_recompute_setting_stable(oracle_mod, st):
+    """With stable=True the recompute backward takes the VJP of the all-pivot
+    lattice, as jax_vjps/hermite.py:78-101 does with the forward's own G.  A
+    displacement of 2 + 2i at (60, 60) makes the vanilla lattice differ from it
+    by about 1e-2, so a backward on the vanilla lattice fails the check."""
+    import torch
+
+    from mrmustard_amd import math, settings
+
+    alpha = 2.0 + 2.0j
+    A = np.array([[0, 1], [1, 0]], complex)
+    b = np.array([alpha, -np.conj(alpha)])
+    c = np.exp(-abs(alpha) ** 2 / 2)
+    shape = (60, 60)
+    rng = np.random.default_rng(78)
+    w = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    _, db, _ = oracle_mod.vanilla_vjp(oracle_mod.stable(shape, A, b, c), c, w)
+    _, db_vanilla, _ = oracle_mod.vanilla_vjp(oracle_mod.vanilla(shape, A, b, c), c, w)
+    assert rel(db_vanilla, db) > 1e-6
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    with settings(HERMITE_RECOMPUTE_VJP=True):
+        G = math.hermite_renormalized(A, bt, c, shape, stable=True)
+    torch.real(torch.sum(torch.tensor(w, device="cuda") * G)).backward()
+    assert rel(np_(bt.grad), np.conj(db)) <= TOL

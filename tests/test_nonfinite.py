@@ -239,6 +239,31 @@ def test_poisoned_cotangent(st, oracle_mod, shape):
         assert same_class_close(np_(x), r)
 
 
+@pytest.mark.parametrize("case", ["blowup-4x80x80", "blowup-3x30x30x40", "poisoned-4x80x80"])
+@pytest.mark.gpu
+@needs_gpu
+def test_recompute_nonfinite(st, oracle_mod, case):
+    """The windowed recompute VJP (slabs too large for one workgroup's LDS) on
+    an overflowing lattice and with a cotangent holding inf and NaN: its
+    non-finite dL/dc sends the call through the stored-lattice path, whose
+    values are the reference's."""
+    kind, dims = case.split("-")
+    shape = tuple(int(x) for x in dims.split("x"))
+    rng = np.random.default_rng(sum(shape) + 11)
+    if kind == "blowup":
+        A, b, c = blowup_triple(sum(shape) + 3, len(shape), 1e4)
+        g = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    else:
+        A, b, c = tame_triple(sum(shape) + 9, len(shape))
+        g = poisoned_cotangent(rng, shape)
+    G = oracle_mod.vanilla(shape, A, b, c)
+    assert has_mix(G) if kind == "blowup" else np.isfinite(G).all()
+    refs = oracle_mod.vanilla_vjp(G, c, g)
+    print(f"recompute non-finite {case}", flush=True)
+    for x, r in zip(st.vanilla_vjp_recompute_hip(shape, A, b, c, g), refs):
+        assert same_class_close(np_(x), r)
+
+
 @pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.gpu
 @needs_gpu
