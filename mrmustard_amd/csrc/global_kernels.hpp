@@ -559,16 +559,19 @@ __global__ __launch_bounds__(1024) void vjp_finalize(const double2 *partial, int
 // that is inside the box.  The row table stores each row's digits (uint16).
 struct StableArgs {
     Geom g;
+    double rstride[8];     // 1 / stride[j], for row digits of lattices below 2^31 points
     const double2 *A, *b, *c;
     double2 *G;
     const uint16_t *rows;  // nrows x 8 digits
-    int64_t nrows;
+    int64_t nrows;         // rows of the last axis: N / shape[D - 1]
     const double *sq, *rsq;
     int n;                 // wavefront (per-wavefront launch)
     int64_t tstride_a, tstride_b, tstride_g;  // batch strides (complex elements)
     int *nf;               // per-wavefront launches: raised when a stored value is not finite
 };
 
+// Digits of every row, read by the per-wavefront and block kernels (a load
+// from L2 is quicker than the row's divisions on their critical path).
 __global__ void row_table_kernel(Geom g, uint16_t *rows, int64_t nrows) {
     const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
@@ -587,15 +590,49 @@ __global__ void row_table_kernel(Geom g, uint16_t *rows, int64_t nrows) {
     }
 }
 
+// Digits k_0..k_{D-2} of row r (the points r * shape[D-1] + k_{D-1}) and their
+// sum, computed (the corner sweeps, once per thread).  Below 2^31 points each quotient is x * (1 / stride) in double
+// precision, off by at most one (the product's error is below 2^-21), then
+// corrected from the remainder.
+template <int D>
+__device__ __forceinline__ int row_digits(const Geom &g, const double (&rinv)[8], int64_t r, int (&k)[D]) {
+    int p = 0;
+    const int64_t f = r * g.shape[D - 1];
+    if (g.N <= int64_t(0x7fffffff)) {
+        int x = int(f);
+#pragma unroll
+        for (int j = 0; j < D - 1; ++j) {
+            const int sj = int(g.stride[j]);
+            int dj = int(double(x) * rinv[j]);
+            const int64_t rem = int64_t(x) - int64_t(dj) * sj;
+            dj += rem >= sj ? 1 : (rem < 0 ? -1 : 0);
+            x -= dj * sj;
+            k[j] = dj;
+            p += dj;
+        }
+    } else {
+        int64_t x = f;
+#pragma unroll
+        for (int j = 0; j < D - 1; ++j) {
+            const int64_t dj = x / g.stride[j];
+            x -= dj * g.stride[j];
+            k[j] = int(dj);
+            p += k[j];
+        }
+    }
+    k[D - 1] = 0;
+    return p;
+}
+
 // Returns true when the stored value is not finite.  BF (branch-free): the
 // compiler can issue the loads of every pivot together; a term of the
 // reference whose weight is sqrt(0) (k_j = 0, or the A_ii term at k_i = 1)
 // is added as the reference adds it, the zero coefficient times a value
 // already computed (the pivot's own), and a pivot axis with k_i = 0 is
 // evaluated on the point's own address and left out of the sum and of the
-// count.  Without BF (fewer registers: the 1024-thread block kernel at
-// D >= 7) zero-weight terms and pivots are skipped, which gives the same
-// values for finite G.
+// count.  Without BF (fewer registers: the 1024-thread kernels at larger D)
+// zero-weight terms and pivots are skipped, which gives the same values for
+// finite G.
 template <int D, bool BF>
 __device__ __forceinline__ bool stable_point(const StableArgs &a, const double2 *A, const double2 *b, double2 *G, const int (&k)[D],
                                              int64_t f) {
@@ -668,6 +705,50 @@ __global__ __launch_bounds__(256) void stable_wavefront_kernel(StableArgs a) {
     const int64_t r0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     bool bad = false;
     for (int64_t r = r0; r < a.nrows; r += int64_t(gridDim.x) * blockDim.x) bad |= stable_row<D, true>(a, a.A, a.b, a.G, r, a.n);
+    if (__builtin_expect(bad, 0) && a.nf) atomicOr(a.nf, 1);
+}
+
+// The rows of a per-wavefront sweep whose digit sum p = k_0 + ... + k_{D-2} is
+// at most plo (list lo) or at least phi (list hi), appended in any order
+// (counts[0..1] start at zero; the host knows both totals, at most cap each).
+template <int D>
+__global__ __launch_bounds__(256) void stable_corner_rows_kernel(StableArgs a, int plo, int phi, int32_t *lo, int32_t *hi, int *counts,
+                                                                 int cap) {
+    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= a.nrows) return;
+    int k[D];
+    const int p = row_digits<D>(a.g, a.rstride, r, k);
+    if (p <= plo) {
+        const int e = atomicAdd(counts, 1);
+        if (e < cap) lo[e] = int32_t(r);
+    }
+    if (p >= phi) {
+        const int e = atomicAdd(counts + 1, 1);
+        if (e < cap) hi[e] = int32_t(r);
+    }
+}
+
+// Wavefronts n0..n1 of one triple in one workgroup, one listed row per
+// thread: the first and last wavefronts of a large lattice hold few points,
+// and a workgroup barrier costs a fraction of a launch.  Every point of these
+// wavefronts lies on a listed row.
+template <int D>
+__global__ __launch_bounds__(1024) void stable_corner_kernel(StableArgs a, const int32_t *rows, int count, int n0, int n1) {
+    const int64_t r = int(threadIdx.x) < count ? rows[threadIdx.x] : -1;
+    const bool has = r >= 0 && r < a.nrows;
+    int k[D];
+    const int p = row_digits<D>(a.g, a.rstride, has ? r : 0, k);
+    const int slast = a.g.shape[D - 1];
+    const int64_t f0 = (has ? r : 0) * slast;
+    bool bad = false;
+    for (int n = n0; n <= n1; ++n) {
+        const int kl = n - p;
+        if (has && kl >= 0 && kl < slast) {
+            k[D - 1] = kl;
+            bad |= stable_point<D, (D <= 4)>(a, a.A, a.b, a.G, k, f0 + kl);
+        }
+        __syncthreads();
+    }
     if (__builtin_expect(bad, 0) && a.nf) atomicOr(a.nf, 1);
 }
 

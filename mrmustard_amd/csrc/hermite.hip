@@ -666,12 +666,43 @@ int launch_finalize(int D, const double2 *partial, int nblk, const double *c, do
 
 // -------------------------------------------------------------- stable path
 
-size_t stable_ws_bytes(const Shape &S) { return size_t(S.N / S.s[S.D - 1]) * 16 + 256; }
+// Rows (of the last axis) listed for the one-workgroup sweeps of a large
+// lattice's first and last wavefronts: one per thread of a 1024-thread workgroup.
+constexpr int64_t STABLE_CORNER_ROWS = 1024;
+
+int64_t stable_list_cap(const Shape &S) { return std::min<int64_t>(S.N / S.s[S.D - 1], STABLE_CORNER_ROWS); }
+
+// The row table (8 digits per row), the counters of two row lists, the lists.
+size_t stable_ws_bytes(const Shape &S) {
+    return align256(size_t(S.N / S.s[S.D - 1]) * 16) + 256 + 2 * align256(size_t(stable_list_cap(S)) * 4);
+}
 
 int stable_check(const Shape &S) {
     for (int j = 0; j + 1 < S.D; ++j)
         if (S.s[j] > 65535) return fail(MM_ERR_UNSUPPORTED, "stable: shape[%d]=%d > 65535", j, S.s[j]);
     return MM_OK;
+}
+
+// Number of rows whose digit sum over axes 0..D-2 is p, for p = 0..sum(shape[j] - 1).
+std::vector<int64_t> row_sum_counts(const Shape &S) {
+    std::vector<int64_t> cnt(1, 1);
+    for (int j = 0; j + 1 < S.D; ++j) {
+        const int64_t sj = S.s[j], m = int64_t(cnt.size());
+        std::vector<int64_t> pre(m + 1, 0);
+        for (int64_t q = 0; q < m; ++q) pre[q + 1] = pre[q] + cnt[q];
+        std::vector<int64_t> nxt(m + sj - 1, 0);
+        for (int64_t p = 0; p < int64_t(nxt.size()); ++p) {
+            const int64_t hi = std::min(p, m - 1), lo = std::max<int64_t>(0, p - sj + 1);
+            nxt[p] = lo <= hi ? pre[hi + 1] - pre[lo] : 0;
+        }
+        cnt.swap(nxt);
+    }
+    return cnt;
+}
+
+template <int D>
+void launch_stable_corner(const StableArgs &a, const int32_t *rows, int count, int n0, int n1, hipStream_t s) {
+    hipLaunchKernelGGL((stable_corner_kernel<D>), dim3(1), dim3(unsigned(STABLE_CORNER_ROWS)), 0, s, a, rows, count, n0, n1);
 }
 
 int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, const double *c, double *G, void *ws, size_t ws_bytes,
@@ -684,43 +715,100 @@ int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, c
     memset(&a, 0, sizeof(a));
     a.g = geom_of(S);
     a.nrows = S.N / S.s[S.D - 1];
-    a.rows = reinterpret_cast<uint16_t *>(ws);
+    char *base = reinterpret_cast<char *>(ws);
+    a.rows = reinterpret_cast<uint16_t *>(base);
+    base += align256(size_t(a.nrows) * 16);
+    hipLaunchKernelGGL(row_table_kernel, dim3(unsigned((a.nrows + 255) / 256)), dim3(256), 0, s, a.g, const_cast<uint16_t *>(a.rows),
+                       a.nrows);
+    LAUNCH_CHECK("row_table_kernel");
+    for (int j = 0; j < S.D; ++j) a.rstride[j] = 1.0 / double(S.st[j]);
     a.sq = ds->sq;
     a.rsq = ds->rsq;
     a.tstride_a = int64_t(S.D) * S.D;
     a.tstride_b = S.D;
     a.tstride_g = S.N;
-    hipLaunchKernelGGL(row_table_kernel, dim3(unsigned((a.nrows + 255) / 256)), dim3(256), 0, s, a.g,
-                       reinterpret_cast<uint16_t *>(ws), a.nrows);
-    LAUNCH_CHECK("row_table_kernel");
     int nmax = 0;
     for (int j = 0; j < S.D; ++j) nmax += S.s[j] - 1;
-    const bool small = S.N <= (int64_t(1) << 18);
-    int *nf = nullptr;
-    if (!small) {
-        rc = nf_flag(s, &nf);
-        if (rc) return rc;
-        a.nf = nf;
-    }
-    for (int64_t t = 0; t < B; ++t) {
-        a.A = reinterpret_cast<const double2 *>(A) + (small ? 0 : t * a.tstride_a);
-        a.b = reinterpret_cast<const double2 *>(b) + (small ? 0 : t * a.tstride_b);
-        a.c = reinterpret_cast<const double2 *>(c) + (small ? 0 : t);
-        a.G = reinterpret_cast<double2 *>(G) + (small ? 0 : t * a.tstride_g);
-        if (small) {
-            switch (S.D) {
+    // Per wavefront, one workgroup per triple loops over ceil(nrows / 1024)
+    // rows per thread, ~1.6 us per row and pivot axis beyond the first (one
+    // round per num_cu triples); a launch over all CUs costs ~5.5 us per triple.
+    const int64_t rounds = (B + ds->num_cu - 1) / ds->num_cu;
+    const double t_block = double((a.nrows + 1023) / 1024) * double(rounds) * 1.6 * std::max(1, S.D - 1);
+    if (t_block <= 5.5 * double(B)) {
+        a.A = reinterpret_cast<const double2 *>(A);
+        a.b = reinterpret_cast<const double2 *>(b);
+        a.c = reinterpret_cast<const double2 *>(c);
+        a.G = reinterpret_cast<double2 *>(G);
+        switch (S.D) {
 #define CASE(DD) \
     case DD: hipLaunchKernelGGL((stable_block_kernel<DD>), dim3(unsigned(B)), dim3(1024), 0, s, a); break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        }
+        LAUNCH_CHECK("stable_block_kernel");
+        return MM_OK;
+    }
+
+    // One launch per wavefront, except, for D <= 3, the first wavefronts
+    // n = 1..nlo (rows with digit sum p <= plo) and the last ones n = nhi..nmax
+    // (rows with p >= phi), each run by one workgroup when their rows fit it.
+    // A point reads D (D + 1) neighbours; from D = 4 on, one CU's vector-memory
+    // pipe makes a workgroup step (one point per thread) cost as much as a launch.
+    int *nf = nullptr;
+    rc = nf_flag(s, &nf);
+    if (rc) return rc;
+    a.nf = nf;
+    const int slast = S.s[S.D - 1];
+    int nlo = 0, nhi = nmax + 1, plo = -1, phi = INT32_MAX;
+    int64_t nlo_rows = 0, nhi_rows = 0;
+    if (a.nrows <= INT32_MAX && S.D <= 3) {
+        const std::vector<int64_t> cnt = row_sum_counts(S);
+        const int P = int(cnt.size()) - 1;
+        while (plo < P && nlo_rows + cnt[plo + 1] <= STABLE_CORNER_ROWS) nlo_rows += cnt[++plo];
+        phi = P + 1;
+        while (phi > 0 && nhi_rows + cnt[phi - 1] <= STABLE_CORNER_ROWS) nhi_rows += cnt[--phi];
+        if (plo == P) {  // every row listed: the first sweep covers every wavefront
+            nlo = nmax;
+        } else {
+            nlo = plo;
+            nhi = std::max(phi + slast - 1, nlo + 1);
+        }
+    }
+    int *counts = reinterpret_cast<int *>(base);
+    int32_t *lo_rows = reinterpret_cast<int32_t *>(base + 256);
+    int32_t *hi_rows = reinterpret_cast<int32_t *>(base + 256 + align256(size_t(stable_list_cap(S)) * 4));
+    if (nlo >= 1 || nhi <= nmax) {
+        hipLaunchKernelGGL(zero_c128_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<double2 *>(counts), int64_t(1));
+        LAUNCH_CHECK("zero_c128_kernel");
+        switch (S.D) {
+#define CASE(DD)                                                                                                                     \
+    case DD:                                                                                                                         \
+        hipLaunchKernelGGL((stable_corner_rows_kernel<DD>), dim3(unsigned((a.nrows + 255) / 256)), dim3(256), 0, s, a, plo,        \
+                           nhi <= nmax ? phi : INT32_MAX, lo_rows, hi_rows, counts, int(stable_list_cap(S)));                        \
+        break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        }
+        LAUNCH_CHECK("stable_corner_rows_kernel");
+    }
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((a.nrows + 255) / 256, int64_t(ds->num_cu) * 16));
+    for (int64_t t = 0; t < B; ++t) {
+        a.A = reinterpret_cast<const double2 *>(A) + t * a.tstride_a;
+        a.b = reinterpret_cast<const double2 *>(b) + t * a.tstride_b;
+        a.c = reinterpret_cast<const double2 *>(c) + t;
+        a.G = reinterpret_cast<double2 *>(G) + t * a.tstride_g;
+        hipLaunchKernelGGL(set_first_kernel, dim3(1), dim3(1), 0, s, a.G, a.c, t == 0 ? nf : nullptr);
+        LAUNCH_CHECK("set_first_kernel");
+        if (nlo >= 1) {
+            switch (S.D) {
+#define CASE(DD) \
+    case DD: launch_stable_corner<DD>(a, lo_rows, int(nlo_rows), 1, nlo, s); break;
                 CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
             }
-            LAUNCH_CHECK("stable_block_kernel");
-            break;  // one launch covers the whole batch
+            LAUNCH_CHECK("stable_corner_kernel");
         }
-        hipLaunchKernelGGL(set_first_kernel, dim3(1), dim3(1), 0, s, a.G, a.c, t == 0 ? nf : nullptr);
-        LAUNCH_CHECK("set_first_kernel");
-        int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((a.nrows + 255) / 256, int64_t(ds->num_cu) * 16));
-        for (int n = 1; n <= nmax; ++n) {
+        for (int n = nlo + 1; n < nhi; ++n) {
             a.n = n;
             switch (S.D) {
 #define CASE(DD) \
@@ -730,10 +818,18 @@ int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, c
             }
             LAUNCH_CHECK("stable_wavefront_kernel");
         }
+        if (nhi <= nmax) {
+            switch (S.D) {
+#define CASE(DD) \
+    case DD: launch_stable_corner<DD>(a, hi_rows, int(nhi_rows), nhi, nmax, s); break;
+                CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+            }
+            LAUNCH_CHECK("stable_corner_kernel");
+        }
     }
-    if (!small)  // non-finite values: the reference's sequential sweep of each flagged call's triples
-        return launch_fixup(S, B, A, b, c, nullptr, G, nullptr, nullptr, nullptr, nf, true, false, ds, s, true);
-    return MM_OK;
+    // non-finite values: the reference's sequential sweep of each flagged call's triples
+    return launch_fixup(S, B, A, b, c, nullptr, G, nullptr, nullptr, nullptr, nf, true, false, ds, s, true);
 }
 
 // The stored lattice's VJP, one slab_fused_kernel launch per level-0 slab
