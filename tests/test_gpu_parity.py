@@ -61,6 +61,8 @@ FWD_SHAPES = [
     (2, 3, 40, 45),
     (2, 4, 4, 4, 4, 4, 4, 4),
     (5, 700), (4, 20, 40), (3, 32, 32),   # level-0 slab of 512..1024 points: one point per thread
+    (5, 50, 50), (20, 45, 50),                        # 2,250-2,500-point slabs of one triple: one workgroup
+    (4, 16, 16, 16), (3, 8, 8, 8, 8), (4, 60, 60), (2, 64, 64),  # 3,600-4,096 points: face in LDS, slabs over all CUs
 ]
 
 

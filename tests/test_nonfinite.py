@@ -165,8 +165,9 @@ def st():
 
 # (shape, scale of b): LDS workgroup sweep, one-point-per-thread sweep,
 # global slabs after an LDS face (a D = 2 and a D = 3 sub-triple's lattice,
-# slabs two per launch), and one-index chains
-BLOWUP = [((60, 60), 1e6), ((30, 30, 30), 1e6), ((4, 80, 80), 1e4), ((3, 30, 30, 40), 1e4), ((300,), 1e4), ((6, 5), 1e80)]
+# slabs two per launch; (6, 60, 60): a slab that fits in LDS, spread over all
+# CUs for a single forward), and one-index chains
+BLOWUP = [((60, 60), 1e6), ((30, 30, 30), 1e6), ((4, 80, 80), 1e4), ((3, 30, 30, 40), 1e4), ((6, 60, 60), 1e6), ((300,), 1e4), ((6, 5), 1e80)]
 
 
 @pytest.mark.parametrize("shape,sb", BLOWUP, ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else f"{v:g}")
