@@ -854,11 +854,25 @@ int launch_vjp_slabs(const Shape &S, const double *G, const double *g, const dou
 // ------------------------------------------------------------ vanilla paths
 
 // Workspace of one vanilla forward (+ fused VJP when `fused`).
-size_t fwd_ws_bytes(const Shape &S, bool fused, const DevState *ds) {
-    if (!fused) return 0;
+// A single triple (D >= 3) whose level-0 slab fits in LDS but holds more
+// than 3072 points (four or more per thread of the workgroup; with the VJP,
+// more than 1024) is swept over all CUs: the face {k_0 = 0} in LDS (team size
+// *Tf), then the level-0 slabs.  Measured for the forward: a workgroup step
+// costs ~1.7 us at three points per thread and ~3.4 us at four, a slab over
+// all CUs ~2.3 us (two per launch); a fused workgroup step adds the VJP terms.
+bool spread_single(const Shape &S, bool fused, int64_t B, const DevState *ds, int *Tf) {
+    if (B != 1 || S.D < 3 || S.st[0] <= (fused ? 1024 : 3072) || !packed_bits_ok(S, 1)) return false;
     int T = 64;
-    int top = (S.D == 1) ? 0 : choose_top(S, true, ds->max_lds, lds_maxt(S.D, true), &T);
-    if (S.D == 1 || top == 0) return 0;
+    if (choose_top(S, fused, ds->max_lds, lds_maxt(S.D, fused), &T) != 0) return false;
+    *Tf = team_threads(S.st[1], lds_maxt(S.D, false));
+    return lds_bytes(S, 1, false, *Tf / MM_WAVE, kernel_dmax(S.D)) <= ds->max_lds;
+}
+
+size_t fwd_ws_bytes(const Shape &S, bool fused, const DevState *ds) {
+    if (!fused || S.D == 1) return 0;
+    int T = 64, Tf = 64;
+    const int top = choose_top(S, true, ds->max_lds, lds_maxt(S.D, true), &T);
+    if (top == 0 && !spread_single(S, true, 1, ds, &Tf)) return 0;
     return vjp_ws_bytes(S, 1);
 }
 
@@ -892,17 +906,10 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
     int T = 64;
     const int maxt = lds_maxt(S.D, fused);
     int top = choose_top(S, fused, ds->max_lds, maxt, &T);
-    // The forward of a single triple (D >= 3) whose level-0 slab fits in LDS but
-    // holds more than 3072 points (four or more per thread of the workgroup)
-    // runs over all CUs: the face {k_0 = 0} in LDS, then the level-0 slabs two
-    // per launch.  Measured: a workgroup step costs ~1.7 us at three points
-    // per thread and ~3.4 us at four, a slab over all CUs ~2.3 us.
-    bool spread = false;
+    // a single triple with large level-0 slabs over all CUs (spread_single);
+    // with the VJP only when the caller gave the VJP workspace for it
     int Tf = 64;
-    if (top == 0 && !fused && B == 1 && S.D >= 3 && S.st[0] > 3072 && packed_bits_ok(S, 1)) {
-        Tf = team_threads(S.st[1], lds_maxt(S.D, false));
-        spread = lds_bytes(S, 1, false, Tf / MM_WAVE, kernel_dmax(S.D)) <= ds->max_lds;
-    }
+    const bool spread = top == 0 && spread_single(S, fused, B, ds, &Tf) && (!fused || (ws && ws_bytes >= vjp_ws_bytes(S, 1)));
     if (top == 0 && !spread) {
         a.tmpl = make_job(S, 0, KIND_BLOCK);
         int64_t shmem = lds_bytes(S, 0, fused, T / MM_WAVE, kernel_dmax(S.D));
@@ -939,10 +946,13 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
         sa.sq = ds->sq;
         sa.rsq = ds->rsq;
         sa.nf = nf;
-        // with the VJP of a single triple, level 0 alternates the VJP terms of
-        // slab m (every row group in one launch) with the forward of slab m + 1,
-        // so that each slab is read while it is still in the caches
-        const bool fuse_level0 = fused && B == 1;
+        // with the VJP of a single triple above 512 MB, level 0 alternates the
+        // VJP terms of slab m (every row group in one launch) with the forward
+        // of slab m + 1, so that each slab is read while it is still in the
+        // caches; below that, the forward (two slabs per launch) then one VJP
+        // pass take fewer launches and measured faster ((60,)^4, 207 MB: 0.82
+        // against 1.12 ms; (10,)^8, 1.6 GB: 6.76 ms interleaved against 6.9)
+        const bool fuse_level0 = fused && B == 1 && S.N * 16 > (int64_t(512) << 20);
         for (int d = top - 1; d >= (fuse_level0 ? 1 : 0); --d) {
             rc = launch_level(sa, S, d, ds, s);
             if (rc) return rc;

@@ -168,9 +168,13 @@ def test_vjp(st, oracle_mod, shape):
         assert rel(np_(x), rs) <= TOL
 
 
-@pytest.mark.parametrize("shape", [(23,), (9, 11), (30, 30, 30), (5, 4, 3, 6), (4, 80, 80), (6, 700), (5, 20, 40), (3, 6, 30, 30, 30)],
+@pytest.mark.parametrize("shape", [(23,), (9, 11), (30, 30, 30), (5, 4, 3, 6), (4, 80, 80), (6, 700), (5, 20, 40), (3, 6, 30, 30, 30),
+                                   (5, 50, 50), (4, 60, 60), (3, 16, 16, 16)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_fused(st, oracle_mod, shape):
+    """One workgroup per triple up to 1,024-point level-0 slabs; above, the
+    face in LDS, the forward's slabs over all CUs, then one VJP pass
+    ((5, 50, 50), (4, 60, 60), (3, 16, 16, 16), (4, 80, 80), D = 5)."""
     rng = np.random.default_rng(sum(shape) + 2000)
     A, b, c = triple(rng, len(shape))
     g = rng.normal(size=shape) + 1j * rng.normal(size=shape)
