@@ -1001,9 +1001,10 @@ int recompute_blocks(const Shape &S, const DevState *ds) { return slab_blocks(S.
 
 
 size_t recompute_ws_bytes(const Shape &S, const DevState *ds) {
-    int T = 64;
+    int T = 64, Tf = 64;
     const int top = (S.D == 1) ? 0 : choose_top(S, true, ds->max_lds, lds_maxt(S.D, true), &T);
-    if (S.D == 1 || top == 0) return align256(size_t(S.N) * 16);  // small: full G, fused sweep
+    if (S.D == 1 || top == 0)  // small: full G, fused sweep (over all CUs with the VJP workspace after G)
+        return align256(size_t(S.N) * 16) + (spread_single(S, true, 1, ds, &Tf) ? vjp_ws_bytes(S, 1) : 0);
     return 3 * align256(size_t(S.st[0]) * 16) + size_t(recompute_blocks(S, ds)) * nacc_of(S.D) * 16 + 256;
 }
 
@@ -1076,8 +1077,11 @@ int vjp_recompute(const Shape &S, const double *A, const double *b, const double
     if (!ws || ws_bytes < need) return fail(MM_ERR_WORKSPACE, "recompute workspace of %zu bytes needed, %zu given", need, ws_bytes);
     int T = 64;
     const int topf = (S.D == 1) ? 0 : choose_top(S, true, ds->max_lds, lds_maxt(S.D, true), &T);
-    if (S.D == 1 || topf == 0)
-        return vanilla_uniform(S, 1, A, b, c, g, reinterpret_cast<double *>(ws), dA, db, dc, nullptr, 0, ds, s);
+    if (S.D == 1 || topf == 0) {
+        const size_t gb = align256(size_t(S.N) * 16);
+        return vanilla_uniform(S, 1, A, b, c, g, reinterpret_cast<double *>(ws), dA, db, dc, reinterpret_cast<char *>(ws) + gb,
+                               ws_bytes - gb, ds, s);
+    }
 
     // three slabs: slab m + 1 is computed into the slot of slab m - 2 once the
     // VJP terms of slab m (the last to read slab m - 2) are launched
