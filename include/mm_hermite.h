@@ -115,9 +115,12 @@ int mm_hermite_batch_fused(const double *A, const double *b, const double *c, co
 /* VJP with recomputation: (dA, db, dc) from (A, b, c, dLdG) without G.  G is
  * recomputed slab by slab (axis-0 slabs) in a window of three slabs held in
  * the workspace, so resident memory is 3 * prod(shape[1:]) complex values
- * instead of prod(shape).  Same results as mm_hermite_vjp applied to
- * mm_hermite_fwd's G (vanilla_vjp_numba, vanilla/gradients.py:25-82, of
- * vanilla_numba's output, core.py:25-124) while every value stays finite.  An
+ * instead of prod(shape).  Lattices of at most 64 MB (4 Mi amplitudes), and
+ * those whose slabs fit in LDS, are computed whole in the workspace instead
+ * (mm_hermite_vjp_recompute_workspace says how much).  Same results as
+ * mm_hermite_vjp applied to mm_hermite_fwd's G (vanilla_vjp_numba,
+ * vanilla/gradients.py:25-82, of vanilla_numba's output, core.py:25-124)
+ * while every value stays finite.  An
  * inf or NaN in G or dLdG makes dc non-finite (dc is the sum of G * dLdG
  * divided by c); the other entries may then differ from the reference's in
  * which parts are NaN or +-inf.  A caller that needs the reference's values

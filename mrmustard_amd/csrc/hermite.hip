@@ -994,17 +994,28 @@ int vanilla_uniform(const Shape &S, int64_t B, const double *A, const double *b,
 // memory: the face k_0 = 0 is computed into window slot 0, then each level-0
 // slab m is recomputed into slot m % 3 from slots (m-1) % 3 and (m-2) % 3 and
 // its VJP terms are added to per-block partial sums.  Resident memory is
-// three slabs (3 * stride[0] * 16 B) instead of the lattice.  Lattices whose
-// slabs fit in LDS take the fused LDS sweep with a full G in the workspace.
+// three slabs (3 * stride[0] * 16 B) instead of the lattice.  Lattices of at
+// most 64 MB, and those whose slabs fit in LDS, take the fused sweep with a
+// full G in the workspace (recompute_whole).
 
 int recompute_blocks(const Shape &S, const DevState *ds) { return slab_blocks(S.st[0], ds); }
 
 
-size_t recompute_ws_bytes(const Shape &S, const DevState *ds) {
+// Lattices held whole in the workspace instead of the window: those whose
+// fused slabs fit in LDS, and any of at most 64 MB (the window's slab-by-slab
+// launches cost more than the memory saves there).  *vjp_ws: the fused path
+// then also needs the VJP workspace, after G.
+bool recompute_whole(const Shape &S, const DevState *ds, bool *vjp_ws) {
     int T = 64, Tf = 64;
     const int top = (S.D == 1) ? 0 : choose_top(S, true, ds->max_lds, lds_maxt(S.D, true), &T);
-    if (S.D == 1 || top == 0)  // small: full G, fused sweep (over all CUs with the VJP workspace after G)
-        return align256(size_t(S.N) * 16) + (spread_single(S, true, 1, ds, &Tf) ? vjp_ws_bytes(S, 1) : 0);
+    const bool whole = S.D == 1 || top == 0 || S.N * 16 <= (int64_t(64) << 20);
+    *vjp_ws = whole && S.D > 1 && (top != 0 || spread_single(S, true, 1, ds, &Tf));
+    return whole;
+}
+
+size_t recompute_ws_bytes(const Shape &S, const DevState *ds) {
+    bool vjp_ws = false;
+    if (recompute_whole(S, ds, &vjp_ws)) return align256(size_t(S.N) * 16) + (vjp_ws ? vjp_ws_bytes(S, 1) : 0);
     return 3 * align256(size_t(S.st[0]) * 16) + size_t(recompute_blocks(S, ds)) * nacc_of(S.D) * 16 + 256;
 }
 
@@ -1075,9 +1086,8 @@ int vjp_recompute(const Shape &S, const double *A, const double *b, const double
                   double *dc, void *ws, size_t ws_bytes, const DevState *ds, hipStream_t s) {
     const size_t need = recompute_ws_bytes(S, ds);
     if (!ws || ws_bytes < need) return fail(MM_ERR_WORKSPACE, "recompute workspace of %zu bytes needed, %zu given", need, ws_bytes);
-    int T = 64;
-    const int topf = (S.D == 1) ? 0 : choose_top(S, true, ds->max_lds, lds_maxt(S.D, true), &T);
-    if (S.D == 1 || topf == 0) {
+    bool vjp_ws = false;
+    if (recompute_whole(S, ds, &vjp_ws)) {
         const size_t gb = align256(size_t(S.N) * 16);
         return vanilla_uniform(S, 1, A, b, c, g, reinterpret_cast<double *>(ws), dA, db, dc, reinterpret_cast<char *>(ws) + gb,
                                ws_bytes - gb, ds, s);

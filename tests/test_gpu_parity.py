@@ -261,13 +261,14 @@ def test_plan_graph_replay(oracle_mod, st):
 
 
 @pytest.mark.parametrize("shape", [(7,), (9, 11), (30, 30, 30), (5, 80, 80), (3, 5, 40, 45), (3, 4, 4, 4, 4, 4, 4, 4), (3, 6, 30, 30, 30),
-                                   (5, 50, 50), (40, 40, 40)],
+                                   (5, 50, 50), (40, 40, 40), (6, 30, 30, 30, 30)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_vjp_recompute(st, oracle_mod, shape):
-    """The windowed recompute VJP equals the VJP of the stored forward; a
-    lattice whose fused slabs fit in LDS is swept whole, in one workgroup
-    ((30, 30, 30)) or, from 1,025-point slabs, over all CUs ((5, 50, 50),
-    (40, 40, 40))."""
+    """The recompute VJP equals the VJP of the stored forward.  Lattices of
+    at most 64 MB are swept whole: in one workgroup ((30, 30, 30)), or from
+    1,025-point slabs over all CUs ((5, 50, 50), (40, 40, 40), and the
+    others up to D = 8); (6, 30, 30, 30, 30), 4.86 M amplitudes (78 MB), goes
+    through the three-slab window."""
     rng = np.random.default_rng(sum(shape) + 5000)
     A, b, c = triple(rng, len(shape))
     g = rng.normal(size=shape) + 1j * rng.normal(size=shape)
