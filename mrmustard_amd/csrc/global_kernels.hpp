@@ -730,8 +730,8 @@ __global__ __launch_bounds__(256) void stable_corner_rows_kernel(StableArgs a, i
 
 // Wavefronts n0..n1 of one triple in one workgroup, one listed row per
 // thread: the first and last wavefronts of a large lattice hold few points,
-// and at D = 2 a workgroup step costs about a third of a launch (at D = 3
-// about as much).  Every point of these wavefronts lies on a listed row.
+// and at D = 2 a workgroup step costs about a third of a launch (used for
+// D = 2 only).  Every point of these wavefronts lies on a listed row.
 template <int D>
 __global__ __launch_bounds__(1024) void stable_corner_kernel(StableArgs a, const int32_t *rows, int count, int n0, int n1) {
     const int64_t r = int(threadIdx.x) < count ? rows[threadIdx.x] : -1;

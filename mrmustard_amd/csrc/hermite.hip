@@ -749,12 +749,12 @@ int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, c
         return MM_OK;
     }
 
-    // One launch per wavefront, except, for D <= 3, the first wavefronts
+    // One launch per wavefront, except, for D = 2, the first wavefronts
     // n = 1..nlo (rows with digit sum p <= plo) and the last ones n = nhi..nmax
     // (rows with p >= phi), each run by one workgroup when their rows fit it.
     // A point reads D (D + 1) neighbours through one CU's vector-memory pipe:
     // measured, a workgroup step (one point per thread) costs ~1.5 us at D = 2,
-    // ~4.6 us at D = 3 (a launch: ~4.7 us) and ~7.3 us at D = 4.
+    // ~4.6 us at D = 3 (a launch in one-wave blocks: ~4.1 us) and ~7.3 us at D = 4.
     int *nf = nullptr;
     rc = nf_flag(s, &nf);
     if (rc) return rc;
@@ -762,7 +762,7 @@ int launch_stable(const Shape &S, int64_t B, const double *A, const double *b, c
     const int slast = S.s[S.D - 1];
     int nlo = 0, nhi = nmax + 1, plo = -1, phi = INT32_MAX;
     int64_t nlo_rows = 0, nhi_rows = 0;
-    if (a.nrows <= INT32_MAX && S.D <= 3) {
+    if (a.nrows <= INT32_MAX && S.D == 2) {
         const std::vector<int64_t> cnt = row_sum_counts(S);
         const int P = int(cnt.size()) - 1;
         while (plo < P && nlo_rows + cnt[plo + 1] <= STABLE_CORNER_ROWS) nlo_rows += cnt[++plo];

@@ -103,10 +103,9 @@ def test_stable_exact(st, oracle_mod, shape):
 def test_stable_exact_large(st, oracle_mod, shape):
     """Bit-exact on both schedules of a large lattice: one workgroup per triple
     ((600, 500): 600 rows of the last axis), and one launch per index-sum
-    wavefront ((40, 40, 40, 5), D = 6, and (4, 4, 4, 4, 40) at D = 5), for
-    D <= 3 with the first and last wavefronts in one workgroup each
-    ((70, 70, 60), and (3500, 9), whose first and last wavefronts fill both
-    1,024-row lists)."""
+    wavefront ((70, 70, 60), (40, 40, 40, 5), D = 6, and (4, 4, 4, 4, 40) at
+    D = 5), at D = 2 with the first and last wavefronts in one workgroup each
+    ((3500, 9), whose first and last wavefronts fill both 1,024-row lists)."""
     rng = np.random.default_rng(sum(shape) + 5)
     A, b, c = triple(rng, len(shape), sym=False)
     print(f"stable large shape={shape}", flush=True)

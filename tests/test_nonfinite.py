@@ -295,17 +295,17 @@ def test_blowup_plan(oracle_mod, st, fused):
 
 
 @pytest.mark.parametrize("shape,sb", [((60, 60), 1e6), ((30, 30, 30), 1e6), ((20, 20), 1e10), ((600, 500), 1e3), ((40, 40, 40), 1e4),
-                                      ((40, 40, 40), 1e8), ((4, 4, 4, 4, 40), 1e8)],
+                                      ((40, 40, 40), 1e8), ((4, 4, 4, 4, 40), 1e8), ((3500, 9), 1e2)],
                          ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else f"{v:g}")
 @pytest.mark.gpu
 @needs_gpu
 def test_stable_blowup(st, oracle_mod, shape, sb):
     """The all-pivot average on overflowing lattices: one workgroup per triple
     (few rows of the last axis), and the per-wavefront sweep ((40, 40, 40):
-    first and last wavefronts in one workgroup, one launch for each wavefront
-    between, overflowing mid-sweep at 1e4 and in the first wavefronts at 1e8;
-    (4, 4, 4, 4, 40): one launch per wavefront at D = 5), then the reference's
-    sequential sweep where a non-finite value appeared."""
+    one launch per wavefront, overflowing mid-sweep at 1e4 and in the first
+    wavefronts at 1e8; (4, 4, 4, 4, 40) at D = 5; (3500, 9): the first and
+    last wavefronts in one workgroup each, overflowing in the first), then the
+    reference's sequential sweep where a non-finite value appeared."""
     A, b, c = blowup_triple(sum(shape) + 5, len(shape), sb)
     ref = oracle_mod.stable(shape, A, b, c)
     assert has_mix(ref)
